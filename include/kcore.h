@@ -1,0 +1,198 @@
+/*
+ * kcore.h -- C ABI of libkcore, the MI355X (gfx950) k-core engine.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No function ever calls
+ * exit(); every failure returns a kc_status and leaves a message readable
+ * with kc_last_error() (thread-local).
+ *
+ * What each group replaces in the reference (chriszhao13/pico):
+ *  - host graph readers/generators  <- Graph::ReadGraph (peel/shared/graph.cpp:17-99),
+ *    VirtualGraph (peel/shared/virtual_graph.cpp:9-183), Graph(string) /
+ *    readFile (peel/src/graph.cpp:58-162)
+ *  - kc_ctx / kc_graph upload       <- the cudaMalloc/cudaMemcpy blocks of each
+ *    main (gppeel.cu:128-176, HistoCore.cu:222-286, ...) and
+ *    malloc_graph_gpu_memory / free_graph_gpu_memory
+ *    (peel/src/gpu_memory_allocation.cu:4-22)
+ *  - kc_decompose                   <- the per-method host loops plus kernels
+ *    (gppeel.cu:186-228, peelone.cu:157-186, peel-dyn.cc:133-182,
+ *    peelone-dyn.cc:120-168, HistoCore.cu:289-341, CntCore.cu:341-417,
+ *    NbrCore.cu:241-297), returning the core vector the reference keeps on
+ *    the device
+ *  - kc_peel_dyn / kc_peelone_dyn   <- `int peel_dyn(Graph&)` (peel-dyn.cc:133) and
+ *    `int peelone_dyn(Graph&)` (peelone-dyn.cc:120), the function pointers
+ *    main.cu:28-29 hands to repSimulation
+ *  - kc_vgraph_decompose            <- the device half of gppeel.cu / peelone.cu /
+ *    HistoCore.cu / CntCore.cu / NbrCore.cu main(), taking the
+ *    degree-prefixed VirtualGraph layout those mains build
+ */
+#ifndef KCORE_H
+#define KCORE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KC_API __attribute__((visibility("default")))
+
+typedef enum kc_status {
+    KC_OK = 0,
+    KC_EINVAL = 1,    /* bad argument or malformed graph */
+    KC_ENOMEM = 2,    /* host or device allocation failed / over budget */
+    KC_EDEVICE = 3,   /* HIP runtime error */
+    KC_EOVERFLOW = 4, /* a device work list overflowed its capacity */
+    KC_ETIMEOUT = 5,  /* a device-side wait exceeded its time limit */
+    KC_EIO = 6        /* file could not be read or written */
+} kc_status;
+
+typedef enum kc_method {
+    KC_GPP = 0,         /* level-synchronous peel, gppeel.cu */
+    KC_PEELONE = 1,     /* PeelOne sweeps, peelone.cu */
+    KC_PEEL_DYN = 2,    /* bucket peel "ICDE23_peel", peel-dyn.cc */
+    KC_PEELONE_DYN = 3, /* bucket peel "Our_Paper_peel", peelone-dyn.cc */
+    KC_HISTO = 4,       /* HistoCore.cu */
+    KC_CNT = 5,         /* CntCore.cu */
+    KC_NBR = 6          /* NbrCore.cu */
+} kc_method;
+
+/* kc_opts.flags bits */
+#define KC_OPT_TRACE 1u /* record a per-phase device trace (kc_graph_trace) */
+
+/* Launch options; zero fields select defaults (or the autotuner's pick). */
+typedef struct kc_opts {
+    uint32_t block;      /* threads per workgroup: 256, 512 or 1024 */
+    uint32_t grid;       /* persistent workgroups (<= occupancy * CUs) */
+    uint32_t hub_split;  /* edges per virtual node of a hub vertex */
+    uint32_t queue_cap;  /* shared work-list capacity in items */
+    uint32_t gpp_compat; /* KC_GPP: report isolated vertices as 1 (reference quirk) */
+    uint32_t part_size;  /* PeelOne reference part width (reporting only) */
+    uint32_t local_cap;  /* soft cap of a workgroup's local frontier; the
+                            excess is shared with every workgroup */
+    uint32_t flags;      /* KC_OPT_* */
+    uint32_t reserved[4];
+} kc_opts;
+
+typedef struct kc_stats {
+    uint64_t iterations; /* the method's "Number of iterations" */
+    uint64_t levels;     /* peel levels 0..kmax (= kmax+1) */
+    uint64_t levels_run; /* levels actually visited after skipping empty ones */
+    uint64_t phases;     /* grid barriers passed */
+    uint64_t edges_walked;
+    uint32_t kmax;
+    uint32_t grid;
+    uint32_t block;
+    uint32_t reserved0;
+    double device_ms; /* HIP-event time of the decomposition on its stream */
+    double total_ms;  /* host wall time of the call incl. copies */
+    uint64_t reserved[4];
+} kc_stats;
+
+KC_API const char *kc_last_error(void);
+KC_API const char *kc_version(void);
+
+/* ------------------------------------------------------------ host graphs */
+typedef struct kc_host_graph kc_host_graph;
+
+typedef struct kc_graph_info {
+    uint32_t V;
+    uint32_t dmax;
+    uint64_t nnz;         /* adjacency entries (2 x undirected edges) */
+    uint64_t edges_read;  /* edge lines kept by a file reader */
+    uint32_t has_zero_id; /* COO reader: some id was 0 (no 1-based shift) */
+    uint32_t multigraph;  /* COO reader keeps duplicate edges */
+    const uint64_t *rowptr; /* owned by the graph */
+    const uint32_t *col;
+} kc_graph_info;
+
+/* reader of gpp/peelone/HistoCore/CntCore/NbrCore (shared/graph.cpp:17-99) */
+KC_API kc_status kc_hg_read_coo(const char *path, kc_host_graph **out);
+/* reader of peelone_dyn (src/graph.cpp:58-133): 1-based, deduplicated */
+KC_API kc_status kc_hg_read_dedup(const char *path, kc_host_graph **out);
+/* symmetrize + drop self-loops + dedup (n directed pairs over V vertices) */
+KC_API kc_status kc_hg_from_pairs(uint32_t V, uint64_t n, const uint32_t *src,
+                                  const uint32_t *dst, kc_host_graph **out);
+/* copy of a symmetric CSR as given (duplicates kept) */
+KC_API kc_status kc_hg_from_csr(uint32_t V, const uint64_t *rowptr, const uint32_t *col,
+                                kc_host_graph **out);
+KC_API kc_status kc_hg_gen_er(uint32_t n, uint64_t m, uint64_t seed, kc_host_graph **out);
+KC_API kc_status kc_hg_gen_rmat(uint32_t scale, uint64_t m_gen, double a, double b, double c,
+                                uint64_t seed, kc_host_graph **out);
+KC_API kc_status kc_hg_gen_chunglu(uint32_t n, double beta, double mean_deg, double cap,
+                                   uint64_t seed, kc_host_graph **out);
+KC_API kc_status kc_hg_info(const kc_host_graph *g, kc_graph_info *info);
+/* degree-prefixed layout of virtual_graph.cpp:93-183: node_pointer[V],
+ * edge_list[nnz+V], parts[2*num_parts] ({node, part} pairs); any output may
+ * be NULL to query *num_parts only */
+KC_API kc_status kc_hg_vgraph(const kc_host_graph *g, uint32_t part_size,
+                              uint32_t *node_pointer, uint32_t *edge_list, uint32_t *parts,
+                              uint64_t *num_parts);
+/* example.txt dialect: 2 header lines, 1-based "u v 1" per undirected edge */
+KC_API kc_status kc_hg_write_edgelist(const kc_host_graph *g, const char *path);
+KC_API void kc_hg_free(kc_host_graph *g);
+
+/* "i core\n" per vertex, the format of utility::SaveResults (util.cpp:70-89) */
+KC_API kc_status kc_write_cores(const char *path, const uint32_t *core, uint32_t V);
+
+/* ---------------------------------------------------------- device side */
+typedef struct kc_ctx kc_ctx;
+typedef struct kc_graph kc_graph;
+
+KC_API kc_status kc_ctx_create(int device, kc_ctx **out);
+KC_API void kc_ctx_destroy(kc_ctx *ctx);
+KC_API kc_status kc_ctx_info(const kc_ctx *ctx, uint32_t *num_cus, uint64_t *hbm_bytes);
+
+/* Copy a symmetric CSR to the device.  rowptr_bytes is 4 or 8: the width the
+ * device keeps (an 8-byte host rowptr may be narrowed when nnz < 2^32).
+ * Inputs are caller-owned and not retained.  Every col[] entry is checked
+ * against V. */
+KC_API kc_status kc_graph_upload(kc_ctx *ctx, uint32_t V, const uint64_t *rowptr,
+                                 const uint32_t *col, uint32_t rowptr_bytes, kc_graph **out);
+KC_API kc_status kc_graph_upload_hg(kc_ctx *ctx, const kc_host_graph *g,
+                                    uint32_t rowptr_bytes, kc_graph **out);
+KC_API void kc_graph_free(kc_graph *g);
+/* device bytes held by the graph (CSR + method scratch allocated so far) */
+KC_API uint64_t kc_graph_device_bytes(const kc_graph *g);
+
+/* Decompose; core_out (host, V entries) may be NULL to leave the result on
+ * the device (kc_graph_core_device returns it). */
+KC_API kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *opts,
+                              uint32_t *core_out, kc_stats *stats);
+KC_API const uint32_t *kc_graph_core_device(const kc_graph *g);
+/* Per-phase trace of the last decomposition run with KC_OPT_TRACE: 8 u64
+ * per phase (start, WG0 done, barrier exit, slowest WG done, level, claims,
+ * shared items, phase kind), times in device ticks (*ticks_per_ms).
+ * Returns the number of phases copied. */
+KC_API uint32_t kc_graph_trace(const kc_graph *g, uint64_t *out, uint32_t max_phases,
+                               uint64_t *ticks_per_ms);
+
+/* ------------------------------------------------- reference-shaped ABI */
+/* Mirrors the data members of the peelone_dyn Graph (peel/inc/graph.h:7-23). */
+typedef struct kc_ref_graph {
+    uint32_t V;
+    uint32_t E; /* adjacency entries, neighbors_offset[V] */
+    uint32_t *neighbors;
+    uint32_t *neighbors_offset;
+    uint32_t *degrees;
+    uint32_t kmax;
+    uint32_t dmax;
+} kc_ref_graph;
+
+/* Drop-in for peel_dyn / peelone_dyn: upload, decompose, free; returns the
+ * elapsed milliseconds (integer, as the reference) and sets g->kmax, or -1
+ * on error.  core_out may be NULL (the reference never copies cores back). */
+KC_API int kc_peel_dyn(kc_ref_graph *g, uint32_t *core_out);
+KC_API int kc_peelone_dyn(kc_ref_graph *g, uint32_t *core_out);
+
+/* Device half of the virtual-graph programs on the degree-prefixed layout
+ * their mains hold after MakeUGraph(): node_pointer[num_nodes],
+ * edge_list[num_edges2 + num_nodes] with edge_list[node_pointer[v]] = deg(v). */
+KC_API kc_status kc_vgraph_decompose(uint32_t num_nodes, uint64_t num_edges2,
+                                     const uint32_t *node_pointer, const uint32_t *edge_list,
+                                     kc_method method, const kc_opts *opts, uint32_t *core_out,
+                                     kc_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

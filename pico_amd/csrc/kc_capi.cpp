@@ -1,0 +1,704 @@
+// kc_capi.cpp -- implementation of the C ABI in include/kcore.h: host graph
+// objects, device contexts and graphs, and dispatch to the gfx950 engines.
+#include "../../include/kcore.h"
+#include "../host/kc_host.hpp"
+#include "kc_engines.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+kc_status fail(kc_status s, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+kc_status hip_fail(hipError_t e, const char *what)
+{
+    return fail(e == hipErrorOutOfMemory ? KC_ENOMEM : KC_EDEVICE, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define KC_HIP(call, what)                                                                         \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return hip_fail(_e, what);                                                             \
+    } while (0)
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+} // namespace
+
+static constexpr uint32_t KC_TRACE_CAP = 1u << 16; // phases kept by the trace
+
+struct kc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t num_cus = 0;
+    uint64_t hbm = 0;
+    uint64_t clock_khz = 100000; // wall_clock64 rate
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct kc_graph {
+    kc_ctx *ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t nnz = 0;
+    bool wide = false;
+    void *rowptr = nullptr;
+    uint32_t *col = nullptr;
+    uint64_t bytes = 0;
+    // method scratch, allocated on first use and kept for reuse
+    DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
+    uint32_t trace_phases = 0;
+    const uint32_t *result = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what)
+{
+    if (b.n >= bytes && b.p)
+        return KC_OK;
+    if (b.p) {
+        hipFree(b.p);
+        g->bytes -= b.n;
+        b.p = nullptr;
+        b.n = 0;
+    }
+    size_t n = bytes ? bytes : 8;
+    hipError_t e = hipMalloc(&b.p, n);
+    if (e != hipSuccess)
+        return hip_fail(e, what);
+    b.n = n;
+    g->bytes += n;
+    return KC_OK;
+}
+
+extern "C" {
+
+const char *kc_last_error(void) { return g_err.c_str(); }
+const char *kc_version(void) { return "kcore-mi355x 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------ host graphs
+kc_status kc_hg_read_coo(const char *path, kc_host_graph **out)
+{
+    if (!path || !out)
+        return fail(KC_EINVAL, "null argument");
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    std::string err;
+    if (kch::read_coo(path, *g, err)) {
+        delete g;
+        return fail(KC_EIO, "%s", err.c_str());
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_read_dedup(const char *path, kc_host_graph **out)
+{
+    if (!path || !out)
+        return fail(KC_EINVAL, "null argument");
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    std::string err;
+    if (kch::read_dedup(path, *g, err)) {
+        delete g;
+        return fail(err.rfind("cannot", 0) == 0 ? KC_EIO : KC_EINVAL, "%s", err.c_str());
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_from_pairs(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *dst,
+                           kc_host_graph **out)
+{
+    if (!out || (n && (!src || !dst)))
+        return fail(KC_EINVAL, "null argument");
+    for (uint64_t i = 0; i < n; i++)
+        if (src[i] >= V || dst[i] >= V)
+            return fail(KC_EINVAL, "pair %llu has an id >= V=%u", (unsigned long long)i, V);
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    kch::csr_from_pairs(V, n, src, dst, *g);
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_from_csr(uint32_t V, const uint64_t *rowptr, const uint32_t *col,
+                         kc_host_graph **out)
+{
+    if (!out || !rowptr || (rowptr[V] && !col))
+        return fail(KC_EINVAL, "null argument");
+    if (rowptr[0] != 0)
+        return fail(KC_EINVAL, "rowptr[0] != 0");
+    for (uint32_t v = 0; v < V; v++)
+        if (rowptr[v + 1] < rowptr[v])
+            return fail(KC_EINVAL, "rowptr decreases at %u", v);
+    for (uint64_t e = 0; e < rowptr[V]; e++)
+        if (col[e] >= V)
+            return fail(KC_EINVAL, "col[%llu]=%u >= V", (unsigned long long)e, col[e]);
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    g->V = V;
+    g->rowptr.assign(rowptr, rowptr + V + 1);
+    g->col.assign(col, col + rowptr[V]);
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_gen_er(uint32_t n, uint64_t m, uint64_t seed, kc_host_graph **out)
+{
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g || !out)
+        return fail(KC_EINVAL, "bad argument");
+    if (kch::gen_er(n, m, seed, *g)) {
+        delete g;
+        return fail(KC_EINVAL, "G(n=%u, m=%llu) is not realisable", n, (unsigned long long)m);
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_gen_rmat(uint32_t scale, uint64_t m_gen, double a, double b, double c,
+                         uint64_t seed, kc_host_graph **out)
+{
+    if (!out || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+        return fail(KC_EINVAL, "bad R-MAT parameters");
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    if (kch::gen_rmat(scale, m_gen, a, b, c, seed, *g)) {
+        delete g;
+        return fail(KC_EINVAL, "R-MAT scale %u out of range", scale);
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t seed,
+                            kc_host_graph **out)
+{
+    if (!out)
+        return fail(KC_EINVAL, "null argument");
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    if (kch::gen_chunglu(n, beta, mean_deg, cap, seed, *g)) {
+        delete g;
+        return fail(KC_EINVAL, "bad Chung-Lu parameters");
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_info(const kc_host_graph *g, kc_graph_info *info)
+{
+    if (!g || !info)
+        return fail(KC_EINVAL, "null argument");
+    info->V = g->V;
+    info->nnz = g->rowptr.empty() ? 0 : g->rowptr[g->V];
+    info->edges_read = g->edges_read;
+    info->has_zero_id = g->has_zero_id;
+    info->multigraph = g->multigraph;
+    uint32_t dmax = 0;
+    for (uint32_t v = 0; v < g->V; v++)
+        dmax = std::max(dmax, (uint32_t)(g->rowptr[v + 1] - g->rowptr[v]));
+    info->dmax = dmax;
+    info->rowptr = g->rowptr.data();
+    info->col = g->col.data();
+    return KC_OK;
+}
+
+kc_status kc_hg_vgraph(const kc_host_graph *g, uint32_t part_size, uint32_t *node_pointer,
+                       uint32_t *edge_list, uint32_t *parts, uint64_t *num_parts)
+{
+    if (!g || part_size == 0)
+        return fail(KC_EINVAL, "bad argument");
+    uint64_t P = 0;
+    for (uint32_t v = 0; v < g->V; v++) {
+        uint64_t d = g->rowptr[v + 1] - g->rowptr[v];
+        uint64_t np = (d + part_size - 1) / part_size;
+        if (node_pointer)
+            node_pointer[v] = (uint32_t)(g->rowptr[v] + v);
+        if (edge_list) {
+            uint64_t b = g->rowptr[v] + v;
+            edge_list[b] = (uint32_t)d;
+            std::memcpy(edge_list + b + 1, g->col.data() + g->rowptr[v], d * sizeof(uint32_t));
+        }
+        if (parts)
+            for (uint64_t j = 0; j < np; j++) {
+                parts[2 * (P + j)] = v;
+                parts[2 * (P + j) + 1] = (uint32_t)j;
+            }
+        P += np;
+    }
+    if (num_parts)
+        *num_parts = P;
+    return KC_OK;
+}
+
+kc_status kc_hg_write_edgelist(const kc_host_graph *g, const char *path)
+{
+    if (!g || !path)
+        return fail(KC_EINVAL, "null argument");
+    std::string err;
+    if (kch::write_edgelist(*g, path, err))
+        return fail(KC_EIO, "%s", err.c_str());
+    return KC_OK;
+}
+
+void kc_hg_free(kc_host_graph *g) { delete g; }
+
+kc_status kc_write_cores(const char *path, const uint32_t *core, uint32_t V)
+{
+    if (!path || (V && !core))
+        return fail(KC_EINVAL, "null argument");
+    FILE *f = std::fopen(path, "wb");
+    if (!f)
+        return fail(KC_EIO, "cannot create %s", path);
+    for (uint32_t i = 0; i < V; i++)
+        std::fprintf(f, "%u %u\n", i, core[i]);
+    std::fclose(f);
+    return KC_OK;
+}
+
+// ---------------------------------------------------------------- device
+kc_status kc_ctx_create(int device, kc_ctx **out)
+{
+    if (!out)
+        return fail(KC_EINVAL, "null argument");
+    int n = 0;
+    KC_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+        return fail(KC_EINVAL, "device %d not present (%d visible)", device, n);
+    KC_HIP(hipSetDevice(device), "hipSetDevice");
+    auto *c = new (std::nothrow) kc_ctx;
+    if (!c)
+        return fail(KC_ENOMEM, "host allocation");
+    c->device = device;
+    hipDeviceProp_t prop;
+    KC_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    c->num_cus = (uint32_t)prop.multiProcessorCount;
+    c->hbm = prop.totalGlobalMem;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->clock_khz = (uint64_t)khz;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    *out = c;
+    return KC_OK;
+}
+
+void kc_ctx_destroy(kc_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream)
+        hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+kc_status kc_ctx_info(const kc_ctx *ctx, uint32_t *num_cus, uint64_t *hbm_bytes)
+{
+    if (!ctx)
+        return fail(KC_EINVAL, "null context");
+    if (num_cus)
+        *num_cus = ctx->num_cus;
+    if (hbm_bytes)
+        *hbm_bytes = ctx->hbm;
+    return KC_OK;
+}
+
+kc_status kc_graph_upload(kc_ctx *ctx, uint32_t V, const uint64_t *rowptr, const uint32_t *col,
+                          uint32_t rowptr_bytes, kc_graph **out)
+{
+    if (!ctx || !rowptr || !out)
+        return fail(KC_EINVAL, "null argument");
+    if (rowptr_bytes != 4 && rowptr_bytes != 8)
+        return fail(KC_EINVAL, "rowptr_bytes must be 4 or 8");
+    if (V == 0xffffffffu)
+        return fail(KC_EINVAL, "V too large");
+    uint64_t nnz = rowptr[V];
+    if (rowptr[0] != 0)
+        return fail(KC_EINVAL, "rowptr[0] != 0");
+    if (nnz && !col)
+        return fail(KC_EINVAL, "null col");
+    for (uint32_t v = 0; v < V; v++)
+        if (rowptr[v + 1] < rowptr[v])
+            return fail(KC_EINVAL, "rowptr decreases at vertex %u", v);
+    for (uint64_t e = 0; e < nnz; e++)
+        if (col[e] >= V)
+            return fail(KC_EINVAL, "col[%llu]=%u out of range (V=%u)", (unsigned long long)e,
+                        col[e], V);
+    bool wide = rowptr_bytes == 8 || nnz > 0xffffffffull;
+    KC_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto *g = new (std::nothrow) kc_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    g->ctx = ctx;
+    g->V = V;
+    g->nnz = nnz;
+    g->wide = wide;
+    size_t rb = (size_t)(V + 1ull) * (wide ? 8 : 4);
+    hipError_t e = hipMalloc(&g->rowptr, rb);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&g->col, nnz ? nnz * 4 : 4);
+    if (e != hipSuccess) {
+        kc_graph_free(g);
+        return hip_fail(e, "graph allocation");
+    }
+    g->bytes = rb + (nnz ? nnz * 4 : 4);
+    if (wide) {
+        e = hipMemcpyAsync(g->rowptr, rowptr, rb, hipMemcpyHostToDevice, ctx->stream);
+    } else {
+        std::vector<uint32_t> r32(V + 1ull);
+        for (uint64_t v = 0; v <= V; v++)
+            r32[v] = (uint32_t)rowptr[v];
+        e = hipMemcpyAsync(g->rowptr, r32.data(), rb, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+    }
+    if (e == hipSuccess && nnz)
+        e = hipMemcpyAsync(g->col, col, nnz * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventCreate(&g->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&g->ev1);
+    if (e != hipSuccess) {
+        kc_graph_free(g);
+        return hip_fail(e, "graph upload");
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_graph_upload_hg(kc_ctx *ctx, const kc_host_graph *hg, uint32_t rowptr_bytes,
+                             kc_graph **out)
+{
+    if (!hg)
+        return fail(KC_EINVAL, "null graph");
+    return kc_graph_upload(ctx, hg->V, hg->rowptr.data(), hg->col.data(), rowptr_bytes, out);
+}
+
+void kc_graph_free(kc_graph *g)
+{
+    if (!g)
+        return;
+    if (g->ctx)
+        hipSetDevice(g->ctx->device);
+    hipFree(g->rowptr);
+    hipFree(g->col);
+    for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
+                      &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
+                      &g->trace})
+        if (b->p)
+            hipFree(b->p);
+    if (g->ev0)
+        hipEventDestroy(g->ev0);
+    if (g->ev1)
+        hipEventDestroy(g->ev1);
+    delete g;
+}
+
+uint64_t kc_graph_device_bytes(const kc_graph *g) { return g ? g->bytes : 0; }
+
+uint32_t kc_graph_trace(const kc_graph *g, uint64_t *out, uint32_t max_phases,
+                        uint64_t *ticks_per_ms)
+{
+    if (!g || !g->trace.p || !g->trace_phases)
+        return 0;
+    uint32_t n = std::min(max_phases, g->trace_phases);
+    if (ticks_per_ms)
+        *ticks_per_ms = g->ctx->clock_khz;
+    if (out && n &&
+        hipMemcpy(out, g->trace.p, (size_t)n * kc::KC_TRACE_WORDS * 8, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return 0;
+    return n;
+}
+
+const uint32_t *kc_graph_core_device(const kc_graph *g) { return g ? g->result : nullptr; }
+
+} // extern "C"
+
+// ----------------------------------------------------------- dispatching
+__global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
+{
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = a[i] > m ? a[i] : m;
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t y = __shfl_xor(m, o, 64);
+        m = y > m ? y : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicMax(out, m);
+}
+
+static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
+{
+    kc_ctx *c = g->ctx;
+    uint32_t block = o.block ? o.block : 1024;
+    if (block != 256 && block != 512 && block != 1024)
+        return fail(KC_EINVAL, "block must be 256, 512 or 1024");
+    int occ = kc::peel_occupancy(block, g->wide);
+    if (occ <= 0)
+        return fail(KC_EDEVICE, "peel kernel cannot be resident with block %u", block);
+    // Persistent grids stay well inside the occupancy limit: every workgroup
+    // must be resident for the grid barrier, and the occupancy query does not
+    // see every per-CU limit (a full 7-per-CU grid of 256-thread groups was
+    // observed not to be co-resident).
+    uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
+    uint32_t maxgrid = per_cu * c->num_cus;
+    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : maxgrid;
+    uint32_t hub = o.hub_split ? o.hub_split : 1024;
+    if (hub < 64 || hub > (1u << 24))
+        return fail(KC_EINVAL, "hub_split must be in [64, 2^24]");
+    uint64_t cap64 = (uint64_t)g->V + g->nnz / hub + 2ull * grid + 64;
+    if (o.queue_cap)
+        cap64 = o.queue_cap;
+    if (cap64 > 0xffffffffull)
+        cap64 = 0xffffffffull;
+    uint32_t cap = (uint32_t)cap64;
+    kc_status s;
+    if ((s = ensure(g, g->deg, (size_t)g->V * 4, "deg")) ||
+        (s = ensure(g, g->done, (size_t)((g->V + 63) / 64) * 8, "claim bitmap")) ||
+        (s = ensure(g, g->list0, (size_t)cap * 8, "work list")) ||
+        (s = ensure(g, g->list1, (size_t)cap * 8, "work list")) ||
+        (s = ensure(g, g->gs, kc::grid_sync_bytes(), "grid sync")) ||
+        (s = ensure(g, g->stats, kc::KC_ST_COUNT * 8, "stats")) ||
+        (s = ensure(g, g->kmax, 8, "kmax")))
+        return s;
+    uint32_t mode = method == KC_GPP ? 2 : method == KC_PEELONE ? 1 : 0;
+    if (mode == 2 && (s = ensure(g, g->out, (size_t)g->V * 4, "gpp out")))
+        return s;
+    kc::PeelLaunch p{};
+    p.rowptr = g->rowptr;
+    p.col = g->col;
+    p.V = g->V;
+    p.hub_split = hub;
+    p.local_cap = o.local_cap ? o.local_cap : 1024;
+    p.deg = (uint32_t *)g->deg.p;
+    p.done = (uint64_t *)g->done.p;
+    p.list0 = (uint64_t *)g->list0.p;
+    p.list1 = (uint64_t *)g->list1.p;
+    p.list_cap = cap;
+    p.mode = mode;
+    p.out = (uint32_t *)g->out.p;
+    p.gs = (kc::GridSync *)g->gs.p;
+    p.stats = (uint64_t *)g->stats.p;
+    p.grid = grid;
+    p.block = block;
+    p.wide = g->wide;
+    p.gpp_compat = o.gpp_compat != 0;
+    p.timeout_ticks = c->clock_khz * 1000ull * 20ull;
+    g->trace_phases = 0;
+    if (o.flags & KC_OPT_TRACE) {
+        if ((s = ensure(g, g->trace, (size_t)KC_TRACE_CAP * kc::KC_TRACE_WORDS * 8, "trace")))
+            return s;
+        KC_HIP(hipMemsetAsync(g->trace.p, 0, g->trace.n, c->stream), "trace clear");
+        p.trace = (uint64_t *)g->trace.p;
+        p.trace_cap = KC_TRACE_CAP;
+    }
+    KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
+    hipError_t e = kc::launch_peel(p, c->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "peel launch");
+    KC_HIP(hipEventRecord(g->ev1, c->stream), "hipEventRecord");
+    KC_HIP(hipMemsetAsync(g->kmax.p, 0, 4, c->stream), "memset");
+    g->result = mode == 2 ? (const uint32_t *)g->out.p : (const uint32_t *)g->deg.p;
+    if (g->V)
+        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(1024, (g->V + 255) / 256)),
+                           dim3(256), 0, c->stream, g->result, g->V, (uint32_t *)g->kmax.p);
+    KC_HIP(hipStreamSynchronize(c->stream), "decomposition");
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    uint64_t sv[kc::KC_ST_COUNT];
+    uint32_t km = 0;
+    KC_HIP(hipMemcpy(sv, g->stats.p, sizeof sv, hipMemcpyDeviceToHost), "stats copy");
+    KC_HIP(hipMemcpy(&km, g->kmax.p, 4, hipMemcpyDeviceToHost), "kmax copy");
+    if (sv[kc::KC_ST_ERR] == 1)
+        return fail(KC_ETIMEOUT, "device wait timed out");
+    if (sv[kc::KC_ST_ERR] == 2)
+        return fail(KC_EOVERFLOW, "shared work list overflow (capacity %u items)", cap);
+    if (sv[kc::KC_ST_ERR])
+        return fail(KC_EDEVICE, "device consistency check failed (%llu)",
+                    (unsigned long long)sv[kc::KC_ST_ERR]);
+    st.device_ms = ms;
+    st.kmax = km;
+    st.levels = g->V ? (uint64_t)km + 1 : 0;
+    st.levels_run = sv[kc::KC_ST_LEVELS_RUN];
+    st.phases = sv[kc::KC_ST_PHASES];
+    st.edges_walked = sv[kc::KC_ST_EDGES];
+    st.grid = grid;
+    st.block = block;
+    if (p.trace)
+        g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
+    if (method == KC_PEEL_DYN || method == KC_PEELONE_DYN)
+        st.iterations = st.levels; // peel-dyn.cc:155-173: one trip per level
+    else
+        st.iterations = sv[kc::KC_ST_ITERS];
+    return KC_OK;
+}
+
+extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *opts,
+                                  uint32_t *core_out, kc_stats *stats)
+{
+    if (!g)
+        return fail(KC_EINVAL, "null graph");
+    double t0 = now_ms();
+    kc_opts o{};
+    if (opts)
+        o = *opts;
+    kc_stats st{};
+    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    kc_status s;
+    switch (method) {
+    case KC_GPP:
+    case KC_PEELONE:
+    case KC_PEEL_DYN:
+    case KC_PEELONE_DYN:
+        s = run_peel(g, method, o, st);
+        break;
+    default:
+        return fail(KC_EINVAL, "method %d not available", (int)method);
+    }
+    if (s)
+        return s;
+    if (core_out && g->V)
+        KC_HIP(hipMemcpy(core_out, g->result, (size_t)g->V * 4, hipMemcpyDeviceToHost),
+               "core copy");
+    st.total_ms = now_ms() - t0;
+    if (stats)
+        *stats = st;
+    return KC_OK;
+}
+
+// ------------------------------------------------- reference-shaped entry
+static std::mutex g_ctx_mu;
+static kc_ctx *g_ctx0 = nullptr;
+
+static kc_ctx *default_ctx()
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (!g_ctx0 && kc_ctx_create(0, &g_ctx0) != KC_OK)
+        g_ctx0 = nullptr;
+    return g_ctx0;
+}
+
+static int ref_peel(kc_ref_graph *rg, uint32_t *core_out, kc_method m)
+{
+    if (!rg || !rg->neighbors_offset || (rg->E && !rg->neighbors)) {
+        fail(KC_EINVAL, "null graph");
+        return -1;
+    }
+    kc_ctx *c = default_ctx();
+    if (!c)
+        return -1;
+    std::vector<uint64_t> rp(rg->V + 1ull);
+    for (uint64_t v = 0; v <= rg->V; v++)
+        rp[v] = rg->neighbors_offset[v];
+    kc_graph *g = nullptr;
+    if (kc_graph_upload(c, rg->V, rp.data(), rg->neighbors, 4, &g) != KC_OK)
+        return -1;
+    // timed like the reference: after the graph is resident, through the
+    // last level (peel-dyn.cc:152-177)
+    auto t0 = std::chrono::steady_clock::now();
+    kc_stats st{};
+    kc_status s = kc_decompose(g, m, nullptr, nullptr, &st);
+    auto t1 = std::chrono::steady_clock::now();
+    if (s == KC_OK && core_out && rg->V &&
+        hipMemcpy(core_out, g->result, (size_t)rg->V * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        s = fail(KC_EDEVICE, "core copy");
+    kc_graph_free(g);
+    if (s)
+        return -1;
+    rg->kmax = st.kmax;
+    return (int)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+}
+
+extern "C" int kc_peel_dyn(kc_ref_graph *g, uint32_t *core_out)
+{
+    return ref_peel(g, core_out, KC_PEEL_DYN);
+}
+
+extern "C" int kc_peelone_dyn(kc_ref_graph *g, uint32_t *core_out)
+{
+    return ref_peel(g, core_out, KC_PEELONE_DYN);
+}
+
+extern "C" kc_status kc_vgraph_decompose(uint32_t num_nodes, uint64_t num_edges2,
+                                         const uint32_t *node_pointer, const uint32_t *edge_list,
+                                         kc_method method, const kc_opts *opts,
+                                         uint32_t *core_out, kc_stats *stats)
+{
+    if (!node_pointer || !edge_list)
+        return fail(KC_EINVAL, "null argument");
+    std::vector<uint64_t> rp(num_nodes + 1ull);
+    std::vector<uint32_t> col(num_edges2);
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < num_nodes; v++) {
+        uint64_t np = node_pointer[v];
+        if (np != pos + v)
+            return fail(KC_EINVAL, "node_pointer[%u] does not follow the degree-prefixed layout", v);
+        uint32_t d = edge_list[np];
+        if (pos + d > num_edges2)
+            return fail(KC_EINVAL, "degree of vertex %u overruns the edge list", v);
+        rp[v] = pos;
+        std::memcpy(col.data() + pos, edge_list + np + 1, (size_t)d * 4);
+        pos += d;
+    }
+    rp[num_nodes] = pos;
+    if (pos != num_edges2)
+        return fail(KC_EINVAL, "degrees sum to %llu, expected %llu", (unsigned long long)pos,
+                    (unsigned long long)num_edges2);
+    kc_ctx *c = default_ctx();
+    if (!c)
+        return KC_EDEVICE;
+    kc_graph *g = nullptr;
+    kc_status s = kc_graph_upload(c, num_nodes, rp.data(), col.data(), 4, &g);
+    if (s)
+        return s;
+    s = kc_decompose(g, method, opts, core_out, stats);
+    kc_graph_free(g);
+    return s;
+}

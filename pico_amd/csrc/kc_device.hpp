@@ -1,0 +1,220 @@
+// kc_device.hpp -- device-side building blocks shared by the gfx950 k-core
+// engines: wave64 helpers, a grid barrier that carries per-phase reductions,
+// bounded spin waits, and the edge-balanced batch walker.
+//
+// Memory-model notes (CDNA3/CDNA4, 8 XCDs with private L2s):
+//  * degree / bitmap updates use HIP's agent-scope relaxed atomics, which are
+//    coherent across XCDs;
+//  * plain loads of those arrays may return stale (larger) values between
+//    barriers; every use below is safe under staleness because degrees only
+//    fall and every decision that needs a fresh value goes through an atomic;
+//  * the grid barrier does release/acquire fences at agent scope, so loads
+//    issued after it see everything written before it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KC_WAVE 64
+
+namespace kc {
+
+// ---------------------------------------------------------------- wave64 ---
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of `mask` strictly below this lane
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t y = __shfl_xor(x, o, KC_WAVE);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        x += __shfl_xor(x, o, KC_WAVE);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        x += __shfl_xor(x, o, KC_WAVE);
+    return x;
+}
+
+// inclusive prefix sum across the wave
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x)
+{
+    uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < KC_WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, KC_WAVE);
+        if (l >= (uint32_t)o)
+            x += y;
+    }
+    return x;
+}
+
+// -------------------------------------------------------------- atomics ---
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_relaxed64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+// ----------------------------------------------------------- grid sync ----
+// Reduction slots carried by the barrier.  Phase g (the work between barrier
+// g-1 and barrier g) accumulates into slot[g & 1]; the last WG to arrive at
+// barrier g clears slot[(g+1) & 1] before releasing, so every slot is clean
+// before its phase starts and stays readable until the following barrier.
+struct PhaseSlot {
+    uint64_t sum0;  // engine-defined sums
+    uint64_t sum1;
+    uint64_t sum2;
+    uint32_t min0;  // engine-defined minimum (cleared to UINT32_MAX)
+    uint32_t qtail; // items pushed to the shared list of this phase
+    uint32_t qhead; // grab counter for reading that list next phase
+    uint32_t flag;  // engine-defined OR
+    uint32_t pad[2];
+};
+
+struct GridSync {
+    uint32_t arrive;
+    uint32_t gen;
+    uint32_t err;      // 0 ok, 1 spin timeout, 2 queue overflow
+    uint32_t nblocks;
+    uint64_t timeout_ticks;
+    uint64_t t0;
+    PhaseSlot slot[2];
+};
+
+__device__ __forceinline__ void slot_clear(PhaseSlot *s)
+{
+    __hip_atomic_store(&s->sum0, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&s->sum1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&s->sum2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st_relaxed(&s->min0, 0xffffffffu);
+    st_relaxed(&s->qtail, 0u);
+    st_relaxed(&s->qhead, 0u);
+    st_relaxed(&s->flag, 0u);
+}
+
+__device__ __forceinline__ bool sync_failed(GridSync *gs)
+{
+    return ld_relaxed(&gs->err) != 0;
+}
+
+__device__ __forceinline__ void sync_fail(GridSync *gs, uint32_t code)
+{
+    atomicCAS(&gs->err, 0u, code);
+}
+
+// Grid-wide barrier.  `gen` is this WG's count of barriers passed so far
+// (identical in every WG).  Returns false if the kernel must abort (timeout
+// or error raised by some WG); all WGs then leave through the same path.
+__device__ __forceinline__ bool grid_barrier(GridSync *gs, uint32_t &gen)
+{
+    __shared__ uint32_t s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        __threadfence();
+        uint32_t nb = gs->nblocks;
+        uint32_t arrived = atomicAdd(&gs->arrive, 1u);
+        if (arrived == nb - 1) {
+            st_relaxed(&gs->arrive, 0u);
+            slot_clear(&gs->slot[(gen + 1) & 1]);
+            __threadfence();
+            atomicAdd(&gs->gen, 1u);
+        } else {
+            uint64_t start = now_ticks();
+            while (ld_relaxed(&gs->gen) == gen) {
+                if (ld_relaxed(&gs->err)) {
+                    ok = 0;
+                    break;
+                }
+                if (now_ticks() - start > gs->timeout_ticks) {
+                    sync_fail(gs, 1);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        if (ld_relaxed(&gs->err))
+            ok = 0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    gen++;
+    return s_ok != 0;
+}
+
+// -------------------------------------------------------- block helpers ---
+// Exclusive scan of `n` (<= blockDim) values held one per thread; returns the
+// thread's exclusive prefix and writes the total to *total (all threads).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total)
+{
+    constexpr int NW = BLOCK / KC_WAVE;
+    __shared__ uint32_t s_w[NW];
+    __shared__ uint32_t s_tot;
+    uint32_t incl = wave_incl_scan_u32(x);
+    uint32_t w = threadIdx.x / KC_WAVE;
+    if (lane_id() == KC_WAVE - 1)
+        s_w[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < NW; i++) {
+            uint32_t t = s_w[i];
+            s_w[i] = run;
+            run += t;
+        }
+        s_tot = run;
+    }
+    __syncthreads();
+    uint32_t r = incl - x + s_w[w];
+    *total = s_tot;
+    return r;
+}
+
+// Upper-bound search: largest i in [0,n) with pre[i] <= e (pre[0] == 0).
+__device__ __forceinline__ uint32_t find_item(const uint32_t *pre, uint32_t n, uint32_t e)
+{
+    uint32_t lo = 0, hi = n; // invariant: pre[lo] <= e < pre[hi] (pre[n] = total)
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+} // namespace kc

@@ -1,0 +1,81 @@
+// kc_engines.hpp -- host-visible launch descriptors of the gfx950 engines.
+// Plain structs of device pointers and sizes; no torch or STL types.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kc {
+
+struct GridSync;
+
+// stats[] slots written by the persistent kernels
+enum {
+    KC_ST_ITERS = 0,      // iteration / round count (method-specific meaning)
+    KC_ST_KMAX = 1,       // max core seen by the kernel
+    KC_ST_PHASES = 2,     // grid barriers passed
+    KC_ST_LEVELS_RUN = 3, // levels actually visited (after skipping)
+    KC_ST_EDGES = 4,      // adjacency entries walked
+    KC_ST_CHANGED = 5,    // h-index: total core-value changes
+    KC_ST_ERR = 6,
+    KC_ST_COUNT = 8
+};
+
+// Optional per-phase trace (profiling aid), KC_TRACE_WORDS u64 per phase:
+//  [0] phase start (WG 0)          [1] WG 0 work done
+//  [2] WG 0 left the barrier       [3] latest work-done time over all WGs
+//  [4] level  [5] claims in phase  [6] items pushed to the shared list
+//  [7] phase kind
+//  [8] most local drain rounds in one WG   [9] edges walked (all WGs)
+//  [10] WG 0 drain rounds                  [11] most edges walked by one WG
+// Times are wall_clock64 ticks.
+enum { KC_TRACE_WORDS = 12 };
+
+struct PeelLaunch {
+    const void *rowptr; // u32[V+1] or u64[V+1] (wide)
+    const uint32_t *col;
+    uint32_t V;
+    uint32_t hub_split;
+    uint32_t local_cap; // soft cap of a workgroup's local frontier
+    uint32_t *deg;  // in/out
+    uint64_t *done; // (V+63)/64 words
+    uint64_t *list0, *list1;
+    uint32_t list_cap;
+    uint32_t mode; // 0 bucket, 1 PeelOne sweeps, 2 GPP
+    uint32_t *out; // GPP final core
+    GridSync *gs;
+    uint64_t *stats;
+    uint32_t grid, block;
+    bool wide;
+    bool gpp_compat;
+    uint64_t timeout_ticks;
+    uint64_t *trace; // may be null
+    uint32_t trace_cap;
+};
+
+hipError_t launch_peel(const PeelLaunch &p, hipStream_t st);
+int peel_occupancy(uint32_t block, bool wide);
+
+struct HindexLaunch {
+    const void *rowptr;
+    const uint32_t *col;
+    uint32_t V;
+    uint32_t method;  // 0 HistoCore, 1 CntCore, 2 NbrCore
+    uint32_t *core;   // out
+    uint32_t *histo;  // rowptr[V]+V words (reference layout)
+    uint32_t *cnt;    // CntCore counters, V words
+    uint32_t *flags;  // 3*V bytes worth of flag words (see kc_hindex.hip)
+    uint32_t *list0, *list1, *list2; // frontier / changed lists, V each
+    uint32_t lds_bins; // per-wave LDS histogram capacity (bins)
+    GridSync *gs;
+    uint64_t *stats;
+    uint32_t grid, block;
+    bool wide;
+    uint64_t timeout_ticks;
+};
+
+hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st);
+int hindex_occupancy(uint32_t block, bool wide);
+
+size_t grid_sync_bytes();
+
+} // namespace kc

@@ -1,0 +1,789 @@
+// kc_peel.hip -- gfx950 peeling engines.
+//
+//  * bucket peel  (peel_dyn / peelone_dyn, peel-dyn.cc:2-182,
+//    peelone-dyn.cc:2-168) and PeelOne sweeps (peelone.cu:14-186): one
+//    persistent launch per decomposition; per level a fused frontier scan +
+//    in-workgroup cascade with clamp-at-level decrements; one grid barrier
+//    per level in the common case.
+//  * GPP level-synchronous peel (gppeel.cu:12-228): same machinery, unclamped
+//    decrements, retire/scatter split so the iteration count equals the
+//    reference's.
+//
+// Device data (all resident in HBM, allocated by kc_capi.cpp):
+//   rowptr[V+1] (u32 or u64), col[nnz] (u32)     -- symmetric CSR, read-only
+//   deg[V]    u32  live degree in, core number out (bucket / PeelOne)
+//   done[]    u64  one claim bit per vertex; a vertex is processed by the
+//                  thread that flips its bit
+//   list0/1   u64  shared work lists (vertex | chunk << 32); hub vertices
+//                  (degree > hub_split) are cut into chunks of hub_split
+//                  edges there so every workgroup can take a share, and
+//                  local-buffer overflow spills there
+//   GridSync       barrier + per-phase reductions
+#include "kc_device.hpp"
+#include "kc_engines.hpp"
+
+namespace kc {
+
+enum { MODE_BUCKET = 0, MODE_PEELONE = 1, MODE_GPP = 2 };
+
+template <typename R> struct PeelArgs {
+    const R *rowptr;
+    const uint32_t *col;
+    uint32_t V;
+    uint32_t hub_split;
+    uint32_t local_cap; // soft cap of the local frontier (<= LQ); excess spills
+    uint32_t *deg;
+    uint64_t *done;
+    uint64_t *list0;
+    uint64_t *list1;
+    uint32_t list_cap;
+    uint32_t mode;
+    uint32_t *out; // GPP: final core
+    GridSync *gs;
+    uint64_t *stats;
+    uint64_t *trace; // optional: KC_TRACE_WORDS per phase, see kc_engines.hpp
+    uint32_t trace_cap;
+};
+
+constexpr uint32_t KC_MAXC = 1024; // live-chunk list capacity per workgroup
+constexpr uint32_t KC_DEAD = 0xffffffffu;
+
+// LDS of one persistent workgroup.
+template <int BLOCK, int LQ> struct PeelLds {
+    uint32_t lbuf[2][LQ]; // local frontier, double buffered
+    uint32_t cnt[2];      // push counters (may exceed LQ: the excess spilled)
+    uint64_t start[BLOCK];
+    uint64_t pre[BLOCK + 1];
+    uint32_t chunks[KC_MAXC]; // this WG's chunks that may still hold live vertices
+    uint32_t nlive;           // entries of chunks[]
+    uint32_t ncw;             // chunks dealt to this WG
+    uint32_t use_list;        // ncw fits in chunks[]
+    uint32_t grab;
+    uint32_t claims; // vertices claimed by this WG in the current phase
+    uint32_t minv;   // min live degree seen by this WG's scan
+    uint32_t live;   // any live vertex seen by the scan
+    uint32_t rounds; // local drain rounds in the current phase (trace only)
+    unsigned long long ph_edges; // edges walked in the current phase (trace only)
+};
+
+template <typename R> __device__ __forceinline__ uint32_t deg0(const R *rp, uint32_t v)
+{
+    return (uint32_t)(rp[v + 1] - rp[v]);
+}
+
+__device__ __forceinline__ uint32_t find_item64(const uint64_t *pre, uint32_t n, uint64_t e)
+{
+    uint32_t lo = 0, hi = n; // pre[lo] <= e < pre[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Wave-aggregated append of the lanes' `want` items to the shared list of
+// this phase.
+template <typename R>
+__device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase, bool want,
+                                            uint64_t item)
+{
+    uint64_t m = ballot(want);
+    if (!m)
+        return;
+    uint32_t lane = lane_id();
+    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    uint64_t *list = (phase & 1) ? a.list1 : a.list0;
+    uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(&slot->qtail, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, KC_WAVE);
+    if (want) {
+        uint32_t p = base + lane_rank(m);
+        if (p < a.list_cap)
+            list[p] = item;
+        else
+            sync_fail(a.gs, 2);
+    }
+}
+
+// Push vertices this wave's lanes claimed.  Bucket/PeelOne: to the local
+// buffer `buf` (spilling to the shared list when full), hubs as chunk items
+// on the shared list; counted as claims.  GPP: crossers go to the shared list
+// whole and are claimed when the next iteration retires them.
+template <int BLOCK, int LQ, typename R>
+__device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, const PeelArgs<R> &a,
+                                            uint32_t phase, bool claim, uint32_t u)
+{
+    uint64_t cm = ballot(claim);
+    if (!cm)
+        return;
+    uint32_t lane = lane_id();
+    if (a.mode == MODE_GPP) {
+        push_shared(a, phase, claim, (uint64_t)u);
+        return;
+    }
+    if (lane == (uint32_t)(__ffsll((unsigned long long)cm) - 1))
+        atomicAdd(&L.claims, (uint32_t)__popcll(cm));
+    uint32_t d0 = claim ? deg0(a.rowptr, u) : 0;
+    bool hub = claim && d0 > a.hub_split;
+    bool loc = claim && !hub;
+    uint64_t lm = ballot(loc);
+    uint32_t idx = 0xffffffffu;
+    if (lm) {
+        uint32_t leader = __ffsll((unsigned long long)lm) - 1;
+        uint32_t base = 0;
+        if (lane == leader)
+            base = atomicAdd(&L.cnt[buf], (uint32_t)__popcll(lm));
+        base = __shfl(base, leader, KC_WAVE);
+        if (loc) {
+            idx = base + lane_rank(lm);
+            if (idx < a.local_cap)
+                L.lbuf[buf][idx] = u;
+        }
+    }
+    push_shared(a, phase, loc && idx >= a.local_cap, (uint64_t)u);
+    if (hub) {
+        PhaseSlot *slot = &a.gs->slot[phase & 1];
+        uint64_t *list = (phase & 1) ? a.list1 : a.list0;
+        uint32_t nch = (d0 + a.hub_split - 1) / a.hub_split;
+        uint32_t base = atomicAdd(&slot->qtail, nch);
+        if (base + nch <= a.list_cap) {
+            for (uint32_t c = 0; c < nch; c++)
+                list[base + c] = (uint64_t)u | ((uint64_t)c << 32);
+        } else {
+            sync_fail(a.gs, 2);
+        }
+    }
+}
+
+// Edge relaxation (inlined in walk_items):
+//  bucket / PeelOne: clamp-at-level decrement (peelone-dyn.cc:92-105,
+//  peelone.cu:340-345): skip u unless deg[u] > level; the lane that moves u
+//  from level+1 onto level wins u's claim bit unless the scan of u's chunk
+//  got there first; a decrement that undershoots is undone.
+//  GPP: unclamped decrement of live vertices (gppeel.cu:92-101); the lane
+//  whose decrement crosses level+1 -> level reports u for the next iteration.
+// Walk the edges of the m items staged in L.start / L.pre (exclusive prefix
+// of their lengths, L.pre[m] = total).  Each thread takes WALK_U edges per
+// step and issues all their loads, then all their atomics, so the memory
+// latency of one step is paid once for WALK_U edges.
+constexpr int WALK_U = 4;
+
+template <int BLOCK, int LQ, typename R>
+__device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                           uint32_t phase, uint32_t m, int pushbuf)
+{
+    const uint64_t total = L.pre[m];
+    for (uint64_t e0 = 0; e0 < total; e0 += (uint64_t)BLOCK * WALK_U) {
+        uint32_t u[WALK_U];
+        bool val[WALK_U];
+#pragma unroll
+        for (int k = 0; k < WALK_U; k++) {
+            uint64_t e = e0 + threadIdx.x + (uint64_t)k * BLOCK;
+            val[k] = e < total;
+            u[k] = 0;
+            if (val[k]) {
+                uint32_t i = find_item64(L.pre, m, e);
+                u[k] = a.col[L.start[i] + (e - L.pre[i])];
+            }
+        }
+        uint64_t pre[WALK_U];
+#pragma unroll
+        for (int k = 0; k < WALK_U; k++) {
+            pre[k] = 0;
+            if (val[k])
+                pre[k] = a.mode == MODE_GPP ? a.done[u[k] >> 6] : (uint64_t)a.deg[u[k]];
+        }
+        uint32_t old[WALK_U];
+        bool hit[WALK_U];
+#pragma unroll
+        for (int k = 0; k < WALK_U; k++) {
+            if (a.mode == MODE_GPP)
+                hit[k] = val[k] && !((pre[k] >> (u[k] & 63)) & 1ull);
+            else
+                hit[k] = val[k] && (uint32_t)pre[k] > level;
+            old[k] = hit[k] ? atomicSub(&a.deg[u[k]], 1u) : 0u;
+        }
+        bool claim[WALK_U];
+#pragma unroll
+        for (int k = 0; k < WALK_U; k++) {
+            claim[k] = false;
+            if (!hit[k])
+                continue;
+            if (a.mode == MODE_GPP) {
+                claim[k] = old[k] == level + 1;
+            } else if (old[k] == level + 1) {
+                if (a.mode != MODE_PEELONE) { // PeelOne: a later sweep fires it
+                    uint64_t bit = 1ull << (u[k] & 63);
+                    uint64_t pv = atomicOr((unsigned long long *)&a.done[u[k] >> 6],
+                                           (unsigned long long)bit);
+                    claim[k] = (pv & bit) == 0;
+                }
+            } else if (old[k] <= level) {
+                atomicAdd(&a.deg[u[k]], 1u); // clamp-at-level restore
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < WALK_U; k++)
+            push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim[k], u[k]);
+    }
+    __syncthreads();
+}
+
+// Block-wide exclusive scan of one u32 length per thread into L.pre.
+template <int BLOCK, int LQ>
+__device__ __forceinline__ void stage_prefix(PeelLds<BLOCK, LQ> &L, uint32_t m, uint32_t len)
+{
+    constexpr int NW = BLOCK / KC_WAVE;
+    __shared__ uint64_t s_w[NW];
+    uint64_t incl = len;
+    uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < KC_WAVE; o <<= 1) {
+        uint64_t y = __shfl_up(incl, o, KC_WAVE);
+        if (l >= (uint32_t)o)
+            incl += y;
+    }
+    uint32_t w = threadIdx.x / KC_WAVE;
+    if (l == KC_WAVE - 1)
+        s_w[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int i = 0; i < NW; i++) {
+            uint64_t t = s_w[i];
+            s_w[i] = run;
+            run += t;
+        }
+        L.pre[m] = run;
+    }
+    __syncthreads();
+    if (threadIdx.x < m)
+        L.pre[threadIdx.x] = incl - len + s_w[w];
+    __syncthreads();
+}
+
+// Process the local frontier until it is empty.  Items claimed while
+// processing buffer b are pushed to buffer b^1.
+template <int BLOCK, int LQ, typename R>
+__device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level, uint32_t phase,
+                      uint64_t &edges)
+{
+    int cur = 0;
+    for (;;) {
+        __syncthreads();
+        uint32_t n = L.cnt[cur];
+        if (n > a.local_cap)
+            n = a.local_cap;
+        if (n == 0)
+            break;
+        for (uint32_t base = 0; base < n; base += BLOCK) {
+            uint32_t m = n - base < (uint32_t)BLOCK ? n - base : (uint32_t)BLOCK;
+            uint32_t len = 0;
+            if (threadIdx.x < m) {
+                uint32_t v = L.lbuf[cur][base + threadIdx.x];
+                R s = a.rowptr[v];
+                len = (uint32_t)(a.rowptr[v + 1] - s);
+                L.start[threadIdx.x] = (uint64_t)s;
+            }
+            edges += len;
+            stage_prefix<BLOCK, LQ>(L, m, len);
+            walk_items<BLOCK, LQ, R>(L, a, level, phase, m, cur ^ 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            L.cnt[cur] = 0;
+            L.rounds++;
+        }
+        cur ^= 1;
+    }
+}
+
+// Frontier scan of this WG's 64-vertex chunks: claim live vertices with
+// deg == level (bucket / PeelOne; peel-dyn.cc:27, peelone.cu:308) or
+// deg <= level (GPP; gppeel.cu:35) and track the minimum live degree for
+// level skipping.  Chunks are dealt round-robin to the workgroups (chunk c
+// belongs to WG c mod grid), so the hubs R-MAT puts at low ids spread out.
+// Each WG keeps the chunks that still hold a live vertex in an LDS list and
+// drops a chunk once its claim word is full.  Every wave keeps four chunks'
+// loads in flight.  Bucket/PeelOne claims go to lbuf[0]; GPP retirees are
+// written out and listed for next phase's scatter.
+template <int BLOCK, int LQ, typename R>
+__device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a,
+                                           uint32_t level, uint32_t phase, uint32_t c,
+                                           uint64_t dw, uint32_t d, bool live, uint32_t &lmin,
+                                           uint32_t &nret)
+{
+    uint32_t lane = lane_id();
+    uint32_t v = c * 64 + lane;
+    bool cand = live && (a.mode == MODE_GPP ? d <= level : d == level);
+    if (live && !cand)
+        lmin = d < lmin ? d : lmin;
+    uint64_t cm = ballot(cand);
+    bool claim = false;
+    if (cm) {
+        uint32_t leader = __ffsll((unsigned long long)cm) - 1;
+        unsigned long long prev = 0;
+        if (lane == leader)
+            prev = atomicOr((unsigned long long *)&a.done[c], (unsigned long long)cm);
+        prev = __shfl(prev, leader, KC_WAVE);
+        claim = ((cm & ~(uint64_t)prev) >> lane) & 1ull;
+    }
+    if (a.mode == MODE_GPP) {
+        if (claim)
+            a.out[v] = level;
+        nret += claim ? 1u : 0u;
+        push_shared(a, phase, claim, (uint64_t)v);
+    } else {
+        push_claims<BLOCK, LQ, R>(L, 0, a, phase, claim, v);
+    }
+}
+
+template <int BLOCK, int LQ, typename R>
+__device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                           uint32_t phase)
+{
+    constexpr uint32_t NW = BLOCK / KC_WAVE;
+    constexpr uint32_t U = 4; // chunks in flight per wave
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / KC_WAVE;
+    const bool use_list = L.use_list != 0;
+    const uint32_t n = use_list ? L.nlive : L.ncw;
+    const uint32_t nb = gridDim.x;
+    uint32_t lmin = 0xffffffffu;
+    uint32_t nret = 0;
+    bool live_seen = false;
+    for (uint32_t j = 0;; j += U) {
+        uint32_t i0 = wave + NW * j;
+        if (i0 >= n)
+            break;
+        uint32_t cc[U];
+        uint64_t dw[U];
+        uint32_t d[U];
+        bool lv[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            uint32_t ik = i0 + NW * k;
+            cc[k] = ik < n ? (use_list ? L.chunks[ik] : blockIdx.x + ik * nb) : KC_DEAD;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++)
+            dw[k] = cc[k] != KC_DEAD ? ld_relaxed64(&a.done[cc[k]]) : ~0ull;
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            uint32_t v = cc[k] * 64 + lane;
+            lv[k] = dw[k] != ~0ull && !((dw[k] >> lane) & 1ull) && v < a.V;
+            d[k] = lv[k] ? a.deg[v] : 0xffffffffu;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            if (cc[k] == KC_DEAD)
+                continue;
+            live_seen |= lv[k];
+            if (dw[k] == ~0ull) {
+                if (use_list && lane == 0)
+                    L.chunks[i0 + NW * k] = KC_DEAD; // every vertex claimed
+                continue;
+            }
+            scan_chunk<BLOCK, LQ, R>(L, a, level, phase, cc[k], dw[k], d[k], lv[k], lmin, nret);
+        }
+    }
+    lmin = wave_min_u32(lmin);
+    uint32_t wr = wave_sum_u32(nret);
+    uint64_t lvm = ballot(live_seen);
+    if (lane == 0) {
+        atomicMin(&L.minv, lmin);
+        if (wr)
+            atomicAdd(&L.claims, wr);
+        if (lvm)
+            L.live = 1;
+    }
+    __syncthreads();
+    if (use_list && wave == 0) { // compact the live-chunk list in place
+        uint32_t out = 0;
+        for (uint32_t b = 0; b < n; b += KC_WAVE) {
+            uint32_t i = b + lane;
+            uint32_t e = i < n ? L.chunks[i] : KC_DEAD;
+            bool keep = e != KC_DEAD;
+            uint64_t m = ballot(keep);
+            if (keep)
+                L.chunks[out + lane_rank(m)] = e;
+            out += (uint32_t)__popcll(m);
+        }
+        if (lane == 0)
+            L.nlive = out;
+    }
+    __syncthreads();
+}
+
+// Consume the previous phase's shared list in grabs of g items.
+//  kind CHUNKS : items are (vertex, chunk) -> edges [chunk*hub, +hub)
+//  kind WHOLE  : items are vertices already retired (GPP scatter)
+//  kind RETIRE : items are GPP crossers, retired here, then scattered
+enum { TAKE_CHUNKS = 0, TAKE_WHOLE = 1, TAKE_RETIRE = 2 };
+
+template <int BLOCK, int LQ, typename R>
+__device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                            uint32_t phase, uint32_t n_items, int kind, uint64_t &edges)
+{
+    PhaseSlot *prev = &a.gs->slot[(phase - 1) & 1];
+    const uint64_t *list = ((phase - 1) & 1) ? a.list1 : a.list0;
+    if (n_items > a.list_cap)
+        n_items = a.list_cap;
+    uint32_t nb = a.gs->nblocks;
+    uint32_t g = (n_items + 2 * nb - 1) / (2 * nb);
+    if (g < 1)
+        g = 1;
+    if (g > (uint32_t)BLOCK)
+        g = BLOCK;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            L.grab = atomicAdd(&prev->qhead, g);
+        __syncthreads();
+        uint32_t s = L.grab;
+        if (s >= n_items)
+            break;
+        uint32_t m = n_items - s < g ? n_items - s : g;
+        uint32_t len = 0;
+        if (threadIdx.x < m) {
+            uint64_t it = list[s + threadIdx.x];
+            uint32_t v = (uint32_t)it;
+            R rs = a.rowptr[v];
+            uint32_t d = (uint32_t)(a.rowptr[v + 1] - rs);
+            if (kind == TAKE_CHUNKS) {
+                uint64_t off = (uint64_t)(uint32_t)(it >> 32) * a.hub_split;
+                uint32_t rem = (uint32_t)(d - off);
+                len = rem < a.hub_split ? rem : a.hub_split;
+                L.start[threadIdx.x] = (uint64_t)rs + off;
+            } else {
+                if (kind == TAKE_RETIRE) {
+                    unsigned long long bit = 1ull << (v & 63);
+                    unsigned long long pv = atomicOr((unsigned long long *)&a.done[v >> 6], bit);
+                    if (!(pv & bit)) {
+                        a.out[v] = level;
+                        atomicAdd(&L.claims, 1u);
+                    }
+                }
+                len = d;
+                L.start[threadIdx.x] = (uint64_t)rs;
+            }
+        }
+        edges += len;
+        stage_prefix<BLOCK, LQ>(L, m, len);
+        walk_items<BLOCK, LQ, R>(L, a, level, phase, m, 0);
+        if (a.mode != MODE_GPP)
+            drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+    }
+}
+
+// phase kinds of the persistent loop
+enum { PH_SCAN = 0, PH_SHARED = 1, PH_GPP_SCATTER = 2, PH_GPP_CROSS = 3 };
+
+template <int BLOCK, int LQ, typename R>
+__global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
+{
+    __shared__ PeelLds<BLOCK, LQ> L;
+    GridSync *gs = a.gs;
+    const uint32_t nb = gridDim.x;
+    {
+        uint32_t nchunks = (a.V + 63) / 64;
+        uint32_t cpw = blockIdx.x < nchunks ? (nchunks - blockIdx.x + nb - 1) / nb : 0;
+        if (threadIdx.x == 0) {
+            L.ncw = cpw;
+            L.use_list = cpw <= KC_MAXC;
+            L.nlive = cpw;
+        }
+        uint32_t nl = cpw < KC_MAXC ? cpw : KC_MAXC;
+        for (uint32_t j = threadIdx.x; j < nl; j += BLOCK)
+            L.chunks[j] = blockIdx.x + j * nb;
+    }
+
+    uint32_t gen = 0;
+    uint32_t level = (a.mode == MODE_GPP) ? 1u : 0u;
+    uint64_t total = 0; // vertices claimed so far (identical in all WGs)
+    uint64_t lvl_claims = 0;
+    uint64_t edges = 0;
+    uint64_t iters = 0;
+    uint64_t levels_run = 0;
+    uint32_t kmax = 0;
+    int kind = PH_SCAN;
+    uint32_t n_prev = 0;
+
+    if (threadIdx.x == 0) {
+        L.cnt[0] = 0;
+        L.cnt[1] = 0;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) {
+            L.claims = 0;
+            L.minv = 0xffffffffu;
+            L.live = 0;
+            L.rounds = 0;
+            L.ph_edges = 0;
+        }
+        __syncthreads();
+        const uint32_t phase = gen;
+        uint64_t *tr = (a.trace && phase < a.trace_cap) ? a.trace + (uint64_t)phase * KC_TRACE_WORDS
+                                                        : nullptr;
+        const uint64_t edges_before = edges;
+        if (tr && blockIdx.x == 0 && threadIdx.x == 0)
+            tr[0] = now_ticks();
+        if (kind == PH_SCAN) {
+            scan_slice<BLOCK, LQ, R>(L, a, level, phase);
+            if (a.mode != MODE_GPP)
+                drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+        } else if (kind == PH_SHARED) {
+            take_shared<BLOCK, LQ, R>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
+        } else {
+            take_shared<BLOCK, LQ, R>(L, a, level, phase, n_prev,
+                                      kind == PH_GPP_CROSS ? TAKE_RETIRE : TAKE_WHOLE, edges);
+        }
+        __syncthreads();
+        if (tr) {
+            uint64_t pe = wave_sum_u64(edges - edges_before);
+            if (lane_id() == 0 && pe)
+                atomicAdd(&L.ph_edges, (unsigned long long)pe);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint64_t t = now_ticks();
+                atomicMax((unsigned long long *)&tr[3], (unsigned long long)t);
+                atomicMax((unsigned long long *)&tr[8], (unsigned long long)L.rounds);
+                atomicAdd((unsigned long long *)&tr[9], L.ph_edges);
+                atomicMax((unsigned long long *)&tr[11], L.ph_edges);
+                if (blockIdx.x == 0) {
+                    tr[1] = t;
+                    tr[10] = L.rounds;
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            PhaseSlot *s = &gs->slot[phase & 1];
+            if (L.claims)
+                atomicAdd((unsigned long long *)&s->sum0, (unsigned long long)L.claims);
+            if (L.minv != 0xffffffffu)
+                atomicMin(&s->min0, L.minv);
+            if (L.live)
+                atomicOr(&s->flag, 1u);
+        }
+        if (!grid_barrier(gs, gen))
+            break;
+        PhaseSlot *s = &gs->slot[phase & 1];
+        uint64_t claims = ld_relaxed64(&s->sum0);
+        uint32_t n_next = ld_relaxed(&s->qtail);
+        uint32_t mn = ld_relaxed(&s->min0);
+        uint32_t live = ld_relaxed(&s->flag);
+        if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
+            tr[2] = now_ticks();
+            tr[4] = level;
+            tr[5] = claims;
+            tr[6] = n_next;
+            tr[7] = (uint64_t)kind;
+        }
+
+        if (a.mode == MODE_GPP) {
+            // iteration accounting of gppeel.cu:186-228
+            if (kind == PH_SCAN) {
+                iters++;
+                if (!live)
+                    break; // the scan saw nothing alive: final iteration
+                total += claims;
+                if (claims == 0) {
+                    // levels level+1 .. mn-1 would each be one empty scan
+                    if (mn == 0xffffffffu || mn <= level) {
+                        sync_fail(gs, 3);
+                        break;
+                    }
+                    iters += (uint64_t)(mn - level - 1);
+                    level = mn;
+                } else {
+                    kind = PH_GPP_SCATTER;
+                    n_prev = n_next;
+                }
+            } else {
+                if (kind == PH_GPP_CROSS) {
+                    iters++;
+                    total += claims;
+                }
+                if (n_next > 0) {
+                    kind = PH_GPP_CROSS;
+                    n_prev = n_next;
+                } else {
+                    iters++; // the empty iteration that ends this level
+                    if (total >= a.V)
+                        break;
+                    level++;
+                    kind = PH_SCAN;
+                }
+            }
+            continue;
+        }
+
+        // bucket peel / PeelOne
+        lvl_claims += claims;
+        if (n_next > 0) {
+            kind = PH_SHARED;
+            n_prev = n_next;
+            continue;
+        }
+        iters++;
+        levels_run++;
+        total += lvl_claims;
+        if (lvl_claims)
+            kmax = level;
+        if (total >= a.V)
+            break;
+        if (a.mode == MODE_PEELONE) {
+            if (lvl_claims == 0)
+                level++;
+        } else if (lvl_claims == 0) {
+            if (mn == 0xffffffffu || mn <= level) {
+                sync_fail(gs, 3);
+                break;
+            }
+            level = mn;
+        } else {
+            level++;
+        }
+        lvl_claims = 0;
+        kind = PH_SCAN;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.stats[KC_ST_ITERS] = iters + (a.mode == MODE_PEELONE ? 1 : 0);
+        a.stats[KC_ST_KMAX] = kmax;
+        a.stats[KC_ST_PHASES] = gen;
+        a.stats[KC_ST_LEVELS_RUN] = levels_run;
+        a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
+    }
+    uint64_t ew = wave_sum_u64(edges);
+    if (lane_id() == 0 && ew)
+        atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], (unsigned long long)ew);
+}
+
+template <typename R>
+__global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uint64_t *done,
+                                 uint32_t ndone, uint32_t *out, GridSync *gs, uint32_t nblocks,
+                                 uint64_t timeout, uint64_t *stats)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = i; v < V; v += stride) {
+        deg[v] = (uint32_t)(rowptr[v + 1] - rowptr[v]);
+        if (out)
+            out[v] = 0;
+    }
+    for (uint32_t w = i; w < ndone; w += stride)
+        done[w] = (w == ndone - 1 && (V & 63)) ? ~((1ull << (V & 63)) - 1) : 0ull;
+    if (i == 0) {
+        gs->arrive = 0;
+        gs->gen = 0;
+        gs->err = 0;
+        gs->nblocks = nblocks;
+        gs->timeout_ticks = timeout;
+        gs->t0 = 0;
+        for (int p = 0; p < 2; p++) {
+            gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
+            gs->slot[p].min0 = 0xffffffffu;
+            gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
+        }
+        for (int k = 0; k < KC_ST_COUNT; k++)
+            stats[k] = 0;
+    }
+}
+
+// True cores for GPP outside compat mode: isolated vertices are 0, not the
+// level-1 value the reference reports (gppeel.cu:182-191).
+template <typename R>
+__global__ void gpp_fix_isolated(const R *rowptr, uint32_t V, uint32_t *out)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && rowptr[v + 1] == rowptr[v])
+        out[v] = 0;
+}
+
+// ------------------------------------------------------------- launchers ---
+template <int BLOCK, int LQ, typename R>
+static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
+{
+    PeelArgs<R> a;
+    a.rowptr = (const R *)p.rowptr;
+    a.col = p.col;
+    a.V = p.V;
+    a.hub_split = p.hub_split;
+    a.local_cap = p.local_cap < (uint32_t)LQ && p.local_cap ? p.local_cap : (uint32_t)LQ;
+    a.deg = p.deg;
+    a.done = p.done;
+    a.list0 = p.list0;
+    a.list1 = p.list1;
+    a.list_cap = p.list_cap;
+    a.mode = p.mode;
+    a.out = p.out;
+    a.gs = p.gs;
+    a.stats = p.stats;
+    a.trace = p.trace;
+    a.trace_cap = p.trace_cap;
+    uint32_t ndone = (p.V + 63) / 64;
+    hipLaunchKernelGGL((peel_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V, p.deg,
+                       p.done, ndone, p.mode == MODE_GPP ? p.out : nullptr, p.gs, p.grid,
+                       p.timeout_ticks, p.stats);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    void *args[] = {&a};
+    e = hipLaunchCooperativeKernel((const void *)peel_kernel<BLOCK, LQ, R>, dim3(p.grid),
+                                   dim3(BLOCK), args, 0, st);
+    if (e != hipSuccess)
+        return e;
+    if (p.mode == MODE_GPP && !p.gpp_compat && p.V)
+        hipLaunchKernelGGL((gpp_fix_isolated<R>), dim3((p.V + 255) / 256), dim3(256), 0, st,
+                           a.rowptr, p.V, p.out);
+    return hipGetLastError();
+}
+
+template <int BLOCK, int LQ> static int occupancy_t(bool wide)
+{
+    int nb = 0;
+    hipError_t e;
+    if (wide)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, peel_kernel<BLOCK, LQ, uint64_t>,
+                                                         BLOCK, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, peel_kernel<BLOCK, LQ, uint32_t>,
+                                                         BLOCK, 0);
+    return e == hipSuccess ? nb : 0;
+}
+
+int peel_occupancy(uint32_t block, bool wide)
+{
+    switch (block) {
+    case 256: return occupancy_t<256, 2048>(wide);
+    case 512: return occupancy_t<512, 4096>(wide);
+    case 1024: return occupancy_t<1024, 4096>(wide);
+    default: return 0;
+    }
+}
+
+hipError_t launch_peel(const PeelLaunch &p, hipStream_t st)
+{
+    switch (p.block) {
+    case 256:
+        return p.wide ? launch_peel_t<256, 2048, uint64_t>(p, st)
+                      : launch_peel_t<256, 2048, uint32_t>(p, st);
+    case 512:
+        return p.wide ? launch_peel_t<512, 4096, uint64_t>(p, st)
+                      : launch_peel_t<512, 4096, uint32_t>(p, st);
+    case 1024:
+        return p.wide ? launch_peel_t<1024, 4096, uint64_t>(p, st)
+                      : launch_peel_t<1024, 4096, uint32_t>(p, st);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+size_t grid_sync_bytes() { return sizeof(GridSync); }
+
+} // namespace kc

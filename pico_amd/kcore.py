@@ -1,0 +1,363 @@
+"""Python view of libkcore's C ABI (include/kcore.h).
+
+The methods keep the reference's names and meanings:
+
+    gpp, peelone, peel_dyn, peelone_dyn      peel family  (peel/*.cu, peel/src/*.cc)
+    HistoCore, CntCore, NbrCore              h-index family (index2core/*.cu)
+
+Every call goes to the HIP library; there is no CPU fallback.  A missing or
+unloadable libkcore.so raises ``KCoreError`` at import of this module's first
+use.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkcore.so")
+
+METHODS = {
+    "gpp": 0, "peelone": 1, "peel_dyn": 2, "peelone_dyn": 3,
+    "histocore": 4, "cntcore": 5, "nbrcore": 6,
+}
+METHOD_NAMES = {v: k for k, v in METHODS.items()}
+STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EOVERFLOW", 5: "ETIMEOUT",
+          6: "EIO"}
+
+
+class KCoreError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("hub_split", ctypes.c_uint32), ("queue_cap", ctypes.c_uint32),
+                ("gpp_compat", ctypes.c_uint32), ("part_size", ctypes.c_uint32),
+                ("local_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("levels", ctypes.c_uint64),
+                ("levels_run", ctypes.c_uint64), ("phases", ctypes.c_uint64),
+                ("edges_walked", ctypes.c_uint64), ("kmax", ctypes.c_uint32),
+                ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if not k.startswith("reserved")}
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [("V", ctypes.c_uint32), ("dmax", ctypes.c_uint32), ("nnz", ctypes.c_uint64),
+                ("edges_read", ctypes.c_uint64), ("has_zero_id", ctypes.c_uint32),
+                ("multigraph", ctypes.c_uint32), ("rowptr", ctypes.POINTER(ctypes.c_uint64)),
+                ("col", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class RefGraph(ctypes.Structure):
+    """Mirror of the peelone_dyn Graph's data members (peel/inc/graph.h:7-23)."""
+    _fields_ = [("V", ctypes.c_uint32), ("E", ctypes.c_uint32),
+                ("neighbors", ctypes.POINTER(ctypes.c_uint32)),
+                ("neighbors_offset", ctypes.POINTER(ctypes.c_uint32)),
+                ("degrees", ctypes.POINTER(ctypes.c_uint32)),
+                ("kmax", ctypes.c_uint32), ("dmax", ctypes.c_uint32)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def lib():
+    """Load libkcore.so (built by __graft_entry__.build / pico_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KCoreError(3, "libkcore.so is not built (%s); run pico_amd.build" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    S = ctypes.c_int
+    sig = {
+        "kc_last_error": ([], ctypes.c_char_p),
+        "kc_version": ([], ctypes.c_char_p),
+        "kc_hg_read_coo": ([ctypes.c_char_p, ctypes.POINTER(_vp)], S),
+        "kc_hg_read_dedup": ([ctypes.c_char_p, ctypes.POINTER(_vp)], S),
+        "kc_hg_from_pairs": ([ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, ctypes.POINTER(_vp)], S),
+        "kc_hg_from_csr": ([ctypes.c_uint32, _vp, _vp, ctypes.POINTER(_vp)], S),
+        "kc_hg_gen_er": ([ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)], S),
+        "kc_hg_gen_rmat": ([ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_vp)], S),
+        "kc_hg_gen_chunglu": ([ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_uint64, ctypes.POINTER(_vp)], S),
+        "kc_hg_info": ([_vp, ctypes.POINTER(GraphInfo)], S),
+        "kc_hg_vgraph": ([_vp, ctypes.c_uint32, _vp, _vp, _vp, _u64p], S),
+        "kc_hg_write_edgelist": ([_vp, ctypes.c_char_p], S),
+        "kc_hg_free": ([_vp], None),
+        "kc_write_cores": ([ctypes.c_char_p, _vp, ctypes.c_uint32], S),
+        "kc_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], S),
+        "kc_ctx_destroy": ([_vp], None),
+        "kc_ctx_info": ([_vp, _u32p, _u64p], S),
+        "kc_graph_upload": ([_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp)], S),
+        "kc_graph_upload_hg": ([_vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp)], S),
+        "kc_graph_free": ([_vp], None),
+        "kc_graph_device_bytes": ([_vp], ctypes.c_uint64),
+        "kc_decompose": ([_vp, ctypes.c_int, ctypes.POINTER(Opts), _vp, ctypes.POINTER(Stats)], S),
+        "kc_graph_core_device": ([_vp], _vp),
+        "kc_peel_dyn": ([ctypes.POINTER(RefGraph), _vp], ctypes.c_int),
+        "kc_peelone_dyn": ([ctypes.POINTER(RefGraph), _vp], ctypes.c_int),
+        "kc_vgraph_decompose": ([ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, ctypes.c_int,
+                                 ctypes.POINTER(Opts), _vp, ctypes.POINTER(Stats)], S),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise KCoreError(status, lib().kc_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None and a.size else None
+
+
+def make_opts(**kw):
+    o = Opts()
+    for k, v in kw.items():
+        if v is not None:
+            setattr(o, k, int(v))
+    return o
+
+
+# ------------------------------------------------------------- host graphs
+class HostGraph:
+    """Host CSR owned by libkcore (kc_host_graph)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = GraphInfo()
+        _check(lib().kc_hg_info(self._h, ctypes.byref(info)))
+        self.V = int(info.V)
+        self.nnz = int(info.nnz)
+        self.dmax = int(info.dmax)
+        self.edges_read = int(info.edges_read)
+        self.has_zero_id = bool(info.has_zero_id)
+        self.multigraph = bool(info.multigraph)
+        self._rp = info.rowptr
+        self._col = info.col
+
+    @property
+    def rowptr(self):
+        return np.ctypeslib.as_array(self._rp, shape=(self.V + 1,)).copy()
+
+    @property
+    def col(self):
+        if self.nnz == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(self._col, shape=(self.nnz,)).copy()
+
+    @property
+    def m(self):
+        return self.nnz // 2
+
+    def vgraph(self, part_size=64):
+        """Degree-prefixed layout of virtual_graph.cpp:93-183."""
+        P = ctypes.c_uint64(0)
+        _check(lib().kc_hg_vgraph(self._h, part_size, None, None, None, ctypes.byref(P)))
+        npn = np.zeros(max(self.V, 1), np.uint32)
+        el = np.zeros(max(self.nnz + self.V, 1), np.uint32)
+        parts = np.zeros(max(2 * P.value, 2), np.uint32)
+        _check(lib().kc_hg_vgraph(self._h, part_size, _ptr(npn), _ptr(el), _ptr(parts),
+                                  ctypes.byref(P)))
+        return npn[:self.V], el[:self.nnz + self.V], parts[:2 * P.value].reshape(-1, 2)
+
+    def write_edgelist(self, path):
+        _check(lib().kc_hg_write_edgelist(self._h, os.fsencode(path)))
+
+    def close(self):
+        if self._h:
+            lib().kc_hg_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _make(fn, *args):
+        h = _vp()
+        _check(fn(*args, ctypes.byref(h)))
+        return HostGraph(h)
+
+    @classmethod
+    def read_coo(cls, path):
+        return cls._make(lib().kc_hg_read_coo, os.fsencode(path))
+
+    @classmethod
+    def read_dedup(cls, path):
+        return cls._make(lib().kc_hg_read_dedup, os.fsencode(path))
+
+    @classmethod
+    def from_pairs(cls, V, src, dst):
+        s = np.ascontiguousarray(src, np.uint32)
+        d = np.ascontiguousarray(dst, np.uint32)
+        return cls._make(lib().kc_hg_from_pairs, V, s.size, _ptr(s), _ptr(d))
+
+    @classmethod
+    def from_csr(cls, rowptr, col):
+        rp = np.ascontiguousarray(rowptr, np.uint64)
+        cl = np.ascontiguousarray(col, np.uint32)
+        return cls._make(lib().kc_hg_from_csr, len(rp) - 1, _ptr(rp), _ptr(cl))
+
+    @classmethod
+    def er(cls, n, m, seed=0):
+        return cls._make(lib().kc_hg_gen_er, n, m, seed)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+        m_gen = int(round(edge_factor * (1 << scale)))
+        return cls._make(lib().kc_hg_gen_rmat, scale, m_gen, a, b, c, seed)
+
+    @classmethod
+    def chunglu(cls, n, beta=2.1, mean_deg=20.0, cap=4e6, seed=3):
+        return cls._make(lib().kc_hg_gen_chunglu, n, beta, mean_deg, cap, seed)
+
+
+# ---------------------------------------------------------------- device
+class Context:
+    def __init__(self, device=0):
+        self._h = _vp()
+        _check(lib().kc_ctx_create(device, ctypes.byref(self._h)))
+        cus = ctypes.c_uint32(0)
+        hbm = ctypes.c_uint64(0)
+        _check(lib().kc_ctx_info(self._h, ctypes.byref(cus), ctypes.byref(hbm)))
+        self.num_cus = cus.value
+        self.hbm_bytes = hbm.value
+
+    def close(self):
+        if self._h:
+            lib().kc_ctx_destroy(self._h)
+            self._h = None
+
+    def upload(self, graph, rowptr_bytes=4):
+        h = _vp()
+        _check(lib().kc_graph_upload_hg(self._h, graph._h, rowptr_bytes, ctypes.byref(h)))
+        return DeviceGraph(self, h, graph.V)
+
+    def upload_csr(self, rowptr, col, rowptr_bytes=4):
+        rp = np.ascontiguousarray(rowptr, np.uint64)
+        cl = np.ascontiguousarray(col, np.uint32)
+        h = _vp()
+        _check(lib().kc_graph_upload(self._h, len(rp) - 1, _ptr(rp), _ptr(cl), rowptr_bytes,
+                                     ctypes.byref(h)))
+        return DeviceGraph(self, h, len(rp) - 1)
+
+
+class DeviceGraph:
+    def __init__(self, ctx, handle, V):
+        self.ctx = ctx
+        self._h = handle
+        self.V = V
+
+    @property
+    def device_bytes(self):
+        return int(lib().kc_graph_device_bytes(self._h))
+
+    def decompose(self, method="peelone_dyn", copy=True, **opts):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        o = make_opts(**opts)
+        st = Stats()
+        core = np.zeros(max(self.V, 1), np.uint32) if copy else None
+        _check(lib().kc_decompose(self._h, m, ctypes.byref(o), _ptr(core), ctypes.byref(st)))
+        return (core[:self.V] if copy else None), st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().kc_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def vgraph_decompose(node_pointer, edge_list, method, **opts):
+    """Device half of the virtual-graph programs on their native layout."""
+    npn = np.ascontiguousarray(node_pointer, np.uint32)
+    el = np.ascontiguousarray(edge_list, np.uint32)
+    V = npn.size
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    core = np.zeros(max(V, 1), np.uint32)
+    st = Stats()
+    o = make_opts(**opts)
+    _check(lib().kc_vgraph_decompose(V, el.size - V, _ptr(npn), _ptr(el), m, ctypes.byref(o),
+                                     _ptr(core), ctypes.byref(st)))
+    return core[:V], st.as_dict()
+
+
+def ref_peel(graph, which="peelone_dyn", want_cores=True):
+    """Drop-in for ``int peel_dyn(Graph&)`` / ``int peelone_dyn(Graph&)``:
+    returns (ms, kmax, cores) for a dedup-reader graph."""
+    rp = np.ascontiguousarray(graph.rowptr, np.uint32)
+    nb = np.ascontiguousarray(graph.col, np.uint32)
+    deg = np.diff(rp).astype(np.uint32)
+    rg = RefGraph()
+    rg.V = graph.V
+    rg.E = int(rp[-1])
+    rg.neighbors = nb.ctypes.data_as(_u32p) if nb.size else None
+    rg.neighbors_offset = rp.ctypes.data_as(_u32p)
+    rg.degrees = deg.ctypes.data_as(_u32p) if deg.size else None
+    rg.dmax = int(deg.max()) if deg.size else 0
+    core = np.zeros(max(graph.V, 1), np.uint32) if want_cores else None
+    fn = lib().kc_peel_dyn if which == "peel_dyn" else lib().kc_peelone_dyn
+    ms = fn(ctypes.byref(rg), _ptr(core))
+    if ms < 0:
+        raise KCoreError(3, lib().kc_last_error().decode(errors="replace"))
+    return ms, int(rg.kmax), (core[:graph.V] if want_cores else None)
+
+
+def _trace_sig():
+    L = lib()
+    if not getattr(L.kc_graph_trace, "_kc_sig", False):
+        L.kc_graph_trace.argtypes = [_vp, _vp, ctypes.c_uint32, _u64p]
+        L.kc_graph_trace.restype = ctypes.c_uint32
+        L.kc_graph_trace._kc_sig = True
+    return L
+
+
+KC_OPT_TRACE = 1
+
+
+def device_trace(dgraph, max_phases=1 << 16):
+    """Per-phase trace of the last decompose(..., flags=KC_OPT_TRACE):
+    a dict of numpy arrays with times in milliseconds."""
+    L = _trace_sig()
+    buf = np.zeros((max_phases, 12), np.uint64)
+    tpm = ctypes.c_uint64(0)
+    n = L.kc_graph_trace(dgraph._h, buf.ctypes.data_as(_vp), max_phases, ctypes.byref(tpm))
+    t = buf[:n].astype(np.float64)
+    f = float(tpm.value or 1)
+    t0 = t[0, 0] if n else 0.0
+    return {
+        "start_ms": (t[:, 0] - t0) / f, "wg0_done_ms": (t[:, 1] - t0) / f,
+        "exit_ms": (t[:, 2] - t0) / f, "last_done_ms": (t[:, 3] - t0) / f,
+        "level": buf[:n, 4].copy(), "claims": buf[:n, 5].copy(),
+        "shared": buf[:n, 6].copy(), "kind": buf[:n, 7].copy(),
+        "max_rounds": buf[:n, 8].copy(), "edges": buf[:n, 9].copy(),
+        "wg0_rounds": buf[:n, 10].copy(), "max_wg_edges": buf[:n, 11].copy(),
+    }
