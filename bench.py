@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""k-core decomposition benchmark on one MI355X.
+
+Workload (the headline configuration): R-MAT scale 22, edge factor 16,
+(a,b,c,d) = (.57,.19,.19,.05), symmetrized, deduplicated, self-loops dropped,
+int32 CSR resident in HBM.  One step = one full decomposition of that graph
+by the plain peel (peelone_dyn engine); HistoCore (h-index) is timed on the
+same graph and reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  value = GTEPS = undirected edges / seconds per step.
+roofline.achieved = algorithmic bytes per launch / mean kernel time, with
+the byte model
+    B_peel = 4(V+1) + 4*2m (col) + 4*2m (degree RMW) + 4V + 4V(kmax+1)
+    B_h    = 4(V+1) + 12*2m + 4(2m+V) + 4V + rounds*V + 4*changed + 16*deg(changed)
+(2m = adjacency entries).  cpu_baseline = the oracle's sequential
+Batagelj-Zaversnik peel on one host core over the same graph.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=float, default=16.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--method", default="peelone_dyn")
+    ap.add_argument("--hindex-method", default="histocore")
+    ap.add_argument("--cpu-runs", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def peel_bytes(V, nnz, kmax, R=4):
+    return R * (V + 1) + 4 * nnz + 4 * nnz + 4 * V + 4 * V * (kmax + 1)
+
+
+def hindex_bytes(V, nnz, rounds, changed, changed_edges, R=4):
+    init = R * (V + 1) + 12 * nnz + 4 * (nnz + V) + 4 * V
+    return init + rounds * V + R * changed + 16 * changed_edges
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def traffic_from_profile(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled against a copy kernel of known size); see
+    profiles/pmc_traffic.json.  None when not measured."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(kernel_key, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs one process on one MI355X "
+              "(see DESIGN.md, 'How the path would shard'); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import pico_amd as pa
+
+    t0 = time.time()
+    g = pa.HostGraph.rmat(args.scale, args.edge_factor, seed=args.seed)
+    gen_s = time.time() - t0
+    V, nnz = g.V, g.nnz
+    m = nnz // 2
+    ctx = pa.Context(0)
+    dg = ctx.upload(g)  # the graph is resident in HBM before timing starts
+
+    # ---------------------------------------------------- plain peel steps
+    for _ in range(args.warmup):
+        dg.decompose(args.method, copy=False)
+    dev_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = dg.decompose(args.method, copy=False)
+        dev_ms.append(st["device_ms"])
+    t_total = time.perf_counter() - t_start
+    ms_per_step = 1e3 * t_total / max(args.steps, 1)
+    core, st = dg.decompose(args.method)
+    kmax = st["kmax"]
+    kernel_ms = float(np.mean(dev_ms))
+    b_alg = peel_bytes(V, nnz, kmax)
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    value = m / (ms_per_step * 1e-3) / 1e9
+
+    # ------------------------------------------------------ h-index (same graph)
+    dg.decompose(args.hindex_method, copy=False)
+    h_ms = []
+    for _ in range(max(3, args.steps // 2)):
+        _, hst = dg.decompose(args.hindex_method, copy=False)
+        h_ms.append(hst["device_ms"])
+    hcore, hst = dg.decompose(args.hindex_method)
+    h_kernel_ms = float(np.mean(h_ms))
+    bh = hindex_bytes(V, nnz, hst["iterations"], hst["changed"], hst["edges_walked"])
+
+    # ------------------------------------------------------- CPU baseline
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        rp, col = g.rowptr, g.col
+        times = []
+        ref = None
+        for _ in range(max(1, args.cpu_runs)):
+            t = time.perf_counter()
+            ref = oracle.bz(rp, col)
+            times.append(time.perf_counter() - t)
+        cpu_s = float(np.mean(times))
+        cpu = {"value": round(m / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": "full R-MAT s22 graph, sequential Batagelj-Zaversnik peel "
+                         "(oracle/kc_oracle.c), mean of %d runs = %.3f s; host %s"
+                         % (len(times), cpu_s, cpu_model())}
+        parity = bool(np.array_equal(core, ref) and np.array_equal(hcore, ref))
+
+    out = {
+        "metric": "k-core GTEPS + % HBM roofline (R-MAT s22); batched vs plain graphs/s; p99",
+        "value": round(value, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded R-MAT generator, pico_amd host C++)",
+        "config": {"workload": "R-MAT scale %d edge factor %g (.57,.19,.19,.05) seed %d: "
+                               "plain peel (%s) on the resident int32 CSR"
+                               % (args.scale, args.edge_factor, args.seed, args.method),
+                   "V": V, "undirected_edges": m, "kmax": kmax,
+                   "levels_run": st["levels_run"], "grid_barriers": st["phases"],
+                   "grid": st["grid"], "block": st["block"], "graph_gen_s": round(gen_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic_from_profile("peel"),
+                     "kernel": "peel_kernel", "kernel_ms": round(kernel_ms, 4),
+                     "algorithmic_bytes": b_alg},
+        "hindex": {"method": args.hindex_method, "kernel_ms": round(h_kernel_ms, 4),
+                   "gteps": round(m / (h_kernel_ms * 1e-3) / 1e9, 4),
+                   "rounds": hst["iterations"], "changed": hst["changed"],
+                   "roofline": {"bound": "hbm",
+                                "achieved": round(bh / (h_kernel_ms * 1e-3) / 1e9, 2),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(bh / (h_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                                "traffic": traffic_from_profile("hindex"),
+                                "kernel": "hindex_kernel", "algorithmic_bytes": bh}},
+        "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
+    }
+    print(json.dumps(out), flush=True)
+    dg.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -78,14 +78,16 @@ typedef struct kc_stats {
     uint64_t levels;     /* peel levels 0..kmax (= kmax+1) */
     uint64_t levels_run; /* levels actually visited after skipping empty ones */
     uint64_t phases;     /* grid barriers passed */
-    uint64_t edges_walked;
+    uint64_t edges_walked; /* peel: adjacency entries walked; HistoCore: adjacency of
+                              changed vertices walked by the update rounds */
     uint32_t kmax;
     uint32_t grid;
     uint32_t block;
     uint32_t reserved0;
     double device_ms; /* HIP-event time of the decomposition on its stream */
     double total_ms;  /* host wall time of the call incl. copies */
-    uint64_t reserved[4];
+    uint64_t changed; /* h-index: core-value changes over all rounds */
+    uint64_t reserved[3];
 } kc_stats;
 
 KC_API const char *kc_last_error(void);
@@ -165,6 +167,10 @@ KC_API const uint32_t *kc_graph_core_device(const kc_graph *g);
  * Returns the number of phases copied. */
 KC_API uint32_t kc_graph_trace(const kc_graph *g, uint64_t *out, uint32_t max_phases,
                                uint64_t *ticks_per_ms);
+/* HistoCore's histogram after the last h-index run, in the reference layout
+ * (HistoCore.cu:265: nnz+V words parallel to the degree-prefixed edge list);
+ * n <= nnz + V words are copied. */
+KC_API kc_status kc_graph_histogram(const kc_graph *g, uint32_t *out, uint64_t n);
 
 /* ------------------------------------------------- reference-shaped ABI */
 /* Mirrors the data members of the peelone_dyn Graph (peel/inc/graph.h:7-23). */

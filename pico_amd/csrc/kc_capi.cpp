@@ -441,6 +441,16 @@ void kc_graph_free(kc_graph *g)
 
 uint64_t kc_graph_device_bytes(const kc_graph *g) { return g ? g->bytes : 0; }
 
+kc_status kc_graph_histogram(const kc_graph *g, uint32_t *out, uint64_t n)
+{
+    if (!g || !out)
+        return fail(KC_EINVAL, "null argument");
+    if (!g->histo.p || n > g->nnz + g->V)
+        return fail(KC_EINVAL, "no histogram of %llu words", (unsigned long long)n);
+    KC_HIP(hipMemcpy(out, g->histo.p, n * 4, hipMemcpyDeviceToHost), "histogram copy");
+    return KC_OK;
+}
+
 uint32_t kc_graph_trace(const kc_graph *g, uint64_t *out, uint32_t max_phases,
                         uint64_t *ticks_per_ms)
 {
@@ -580,6 +590,88 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     return KC_OK;
 }
 
+static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
+{
+    kc_ctx *c = g->ctx;
+    uint32_t block = o.block ? o.block : 512;
+    if (block != 256 && block != 512 && block != 1024)
+        return fail(KC_EINVAL, "block must be 256, 512 or 1024");
+    int occ = kc::hindex_occupancy(block, g->wide);
+    if (occ <= 0)
+        return fail(KC_EDEVICE, "h-index kernel cannot be resident with block %u", block);
+    uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
+    uint32_t maxgrid = per_cu * c->num_cus;
+    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : maxgrid;
+    size_t V4 = (size_t)(g->V ? g->V : 1) * 4;
+    kc_status s;
+    if ((s = ensure(g, g->deg, V4, "core")) ||
+        (s = ensure(g, g->histo, (size_t)(g->nnz + g->V + 1) * 4, "histogram")) ||
+        (s = ensure(g, g->hl0, V4, "list")) || (s = ensure(g, g->hl1, V4, "list")) ||
+        (s = ensure(g, g->gs, kc::grid_sync_bytes(), "grid sync")) ||
+        (s = ensure(g, g->stats, kc::KC_ST_COUNT * 8, "stats")) ||
+        (s = ensure(g, g->kmax, 8, "kmax")))
+        return s;
+    if (method != KC_HISTO &&
+        ((s = ensure(g, g->out, V4, "new values")) || (s = ensure(g, g->flags, V4, "marks")) ||
+         (s = ensure(g, g->hl2, V4, "list"))))
+        return s;
+    if (method == KC_CNT &&
+        ((s = ensure(g, g->cnt, V4, "cnt")) || (s = ensure(g, g->list0, V4, "list"))))
+        return s;
+    kc::HindexLaunch p{};
+    p.rowptr = g->rowptr;
+    p.col = g->col;
+    p.V = g->V;
+    p.method = method == KC_HISTO ? 0 : method == KC_CNT ? 1 : 2;
+    p.core = (uint32_t *)g->deg.p;
+    p.histo = (uint32_t *)g->histo.p;
+    p.cnt = (uint32_t *)g->cnt.p;
+    p.newc = (uint32_t *)g->out.p;
+    p.mark = (uint32_t *)g->flags.p;
+    p.list0 = (uint32_t *)g->hl0.p;
+    p.list1 = (uint32_t *)g->hl1.p;
+    p.list2 = (uint32_t *)g->hl2.p;
+    p.list3 = (uint32_t *)g->list0.p;
+    p.gs = (kc::GridSync *)g->gs.p;
+    p.stats = (uint64_t *)g->stats.p;
+    p.grid = grid;
+    p.block = block;
+    p.wide = g->wide;
+    p.timeout_ticks = c->clock_khz * 1000ull * 20ull;
+    KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
+    hipError_t e = kc::launch_hindex(p, c->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "h-index launch");
+    KC_HIP(hipEventRecord(g->ev1, c->stream), "hipEventRecord");
+    KC_HIP(hipMemsetAsync(g->kmax.p, 0, 4, c->stream), "memset");
+    g->result = (const uint32_t *)g->deg.p;
+    if (g->V)
+        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(1024, (g->V + 255) / 256)),
+                           dim3(256), 0, c->stream, g->result, g->V, (uint32_t *)g->kmax.p);
+    KC_HIP(hipStreamSynchronize(c->stream), "decomposition");
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    uint64_t sv[kc::KC_ST_COUNT];
+    uint32_t km = 0;
+    KC_HIP(hipMemcpy(sv, g->stats.p, sizeof sv, hipMemcpyDeviceToHost), "stats copy");
+    KC_HIP(hipMemcpy(&km, g->kmax.p, 4, hipMemcpyDeviceToHost), "kmax copy");
+    if (sv[kc::KC_ST_ERR] == 1)
+        return fail(KC_ETIMEOUT, "device wait timed out");
+    if (sv[kc::KC_ST_ERR])
+        return fail(KC_EOVERFLOW, "h-index list overflow (%llu)",
+                    (unsigned long long)sv[kc::KC_ST_ERR]);
+    st.device_ms = ms;
+    st.kmax = km;
+    st.levels = g->V ? (uint64_t)km + 1 : 0;
+    st.iterations = sv[kc::KC_ST_ITERS];
+    st.phases = sv[kc::KC_ST_PHASES];
+    st.edges_walked = sv[kc::KC_ST_EDGES];
+    st.changed = sv[kc::KC_ST_CHANGED];
+    st.grid = grid;
+    st.block = block;
+    return KC_OK;
+}
+
 extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *opts,
                                   uint32_t *core_out, kc_stats *stats)
 {
@@ -598,6 +690,11 @@ extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *
     case KC_PEEL_DYN:
     case KC_PEELONE_DYN:
         s = run_peel(g, method, o, st);
+        break;
+    case KC_HISTO:
+    case KC_CNT:
+    case KC_NBR:
+        s = run_hindex(g, method, o, st);
         break;
     default:
         return fail(KC_EINVAL, "method %d not available", (int)method);

@@ -63,9 +63,9 @@ struct HindexLaunch {
     uint32_t *core;   // out
     uint32_t *histo;  // rowptr[V]+V words (reference layout)
     uint32_t *cnt;    // CntCore counters, V words
-    uint32_t *flags;  // 3*V bytes worth of flag words (see kc_hindex.hip)
-    uint32_t *list0, *list1, *list2; // frontier / changed lists, V each
-    uint32_t lds_bins; // per-wave LDS histogram capacity (bins)
+    uint32_t *newc;   // deferred new values, V words
+    uint32_t *mark;   // epoch stamps, V words
+    uint32_t *list0, *list1, *list2, *list3; // frontier / changed lists, V each
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;

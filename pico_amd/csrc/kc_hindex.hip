@@ -1,0 +1,714 @@
+// kc_hindex.hip -- gfx950 h-index ("index2core") engines: HistoCore, CntCore
+// and NbrCore (index2core/HistoCore.cu, CntCore.cu, NbrCore.cu).
+//
+// All three iterate c <- min(c, H(c)) from c = degree, where H(c)(v) is the
+// h-index of the neighbours' current values; they differ in which vertices
+// are recomputed each round and how.  One persistent launch runs every
+// round; grid barriers separate the phases of a round, so every method keeps
+// the reference's Jacobi round structure and its "Number of iterations".
+//
+//  HistoCore (HistoCore.cu:15-202): a per-vertex histogram kept in HBM,
+//    laid out parallel to the degree-prefixed edge list (bins of v start at
+//    rowptr[v]+v, deg(v)+1 words); a round scans bins of the frontier from
+//    the old core down, collapses them, and patches neighbours' bins with two
+//    atomics per affected edge, flagging a neighbour whose top bin drops
+//    below its core.
+//  NbrCore (NbrCore.cu:12-167): recompute the frontier's h-index from
+//    scratch; next frontier = neighbours of changed vertices.
+//  CntCore (CntCore.cu:17-264): recompute vertices with cnt < core; recount
+//    cnt (neighbours with core >= own core) around changed vertices.
+//
+// Work mapping: one wavefront per vertex, its histogram in LDS when it fits
+// (bins capped at the vertex's current value, which bounds the result), in
+// the vertex's HBM bin region otherwise; vertices above KC_HBIG neighbours
+// (or bins) are handed to the whole workgroup.  Lists of frontier / changed
+// vertices are compacted with wave-aggregated atomics; round-1 sweeps run
+// over the vertex range directly.
+#include "kc_device.hpp"
+#include "kc_engines.hpp"
+
+namespace kc {
+
+enum { HM_HISTO = 0, HM_CNT = 1, HM_NBR = 2 };
+
+constexpr uint32_t KC_LBINS = 512;  // LDS histogram bins per wavefront
+constexpr uint32_t KC_HBIG = 4096;  // items above this go to the workgroup
+constexpr uint32_t KC_HGRAB = 16;   // items taken per wavefront grab
+constexpr uint32_t KC_BIGQ = 256;   // per-WG queue of big items
+
+template <typename R> struct HArgs {
+    const R *rowptr;
+    const uint32_t *col;
+    uint32_t V;
+    uint32_t method;
+    uint32_t *core;
+    uint32_t *histo; // rowptr[V] + V words
+    uint32_t *cnt;   // CntCore
+    uint32_t *newc;  // deferred values (NbrCore / CntCore)
+    uint32_t *mark;  // epoch stamps (NbrCore / CntCore)
+    uint32_t *list[4];
+    GridSync *gs;
+    uint64_t *stats;
+};
+
+template <int BLOCK> struct HLds {
+    uint32_t bins[BLOCK / KC_WAVE][KC_LBINS];
+    uint32_t bigq[KC_BIGQ];
+    uint32_t nbig;
+    uint32_t grab;
+    uint32_t red[BLOCK / KC_WAVE];
+    uint32_t bk, bsum; // block-level h-index result
+    uint32_t done;
+};
+
+template <typename R> __device__ __forceinline__ uint64_t hbase(const R *rp, uint32_t v)
+{
+    return (uint64_t)rp[v] + v;
+}
+
+// Wave-aggregated append to list `lst` whose counter is this phase's slot.
+__device__ __forceinline__ void hpush(GridSync *gs, uint32_t phase, uint32_t *lst, uint32_t cap,
+                                      bool want, uint32_t v)
+{
+    uint64_t m = ballot(want);
+    if (!m)
+        return;
+    uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(&gs->slot[phase & 1].qtail, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, KC_WAVE);
+    if (want) {
+        uint32_t p = base + lane_rank(m);
+        if (p < cap)
+            lst[p] = v;
+        else
+            sync_fail(gs, 2);
+    }
+}
+
+// Top-down search over bins[1..top] for the largest k <= top with
+// sum_{j>=k} bins[j] >= k.  Returns k (0 if none) and the suffix sum at k.
+// `global_bins` selects coherent loads for HBM bins.
+__device__ __forceinline__ uint32_t wave_hsearch(const uint32_t *bins, uint32_t top, uint32_t &sum)
+{
+    uint32_t lane = lane_id();
+    uint32_t carry = 0;
+    for (int64_t hi = top; hi >= 1; hi -= KC_WAVE) {
+        int64_t j = hi - lane;
+        bool valid = j >= 1;
+        uint32_t b = valid ? bins[j] : 0;
+        uint32_t s = carry + wave_incl_scan_u32(b);
+        bool ok = valid && s >= (uint32_t)j;
+        uint64_t m = ballot(ok);
+        if (m) {
+            uint32_t f = __ffsll((unsigned long long)m) - 1;
+            sum = __shfl(s, f, KC_WAVE);
+            return (uint32_t)(hi - f);
+        }
+        carry = __shfl(s, KC_WAVE - 1, KC_WAVE);
+    }
+    sum = carry;
+    return 0;
+}
+
+// Histogram of min(cap, val(u)) over u in N(v) into bins[0..cap].  LDS or
+// HBM bins; the caller fences afterwards.
+template <typename R, bool LDSB>
+__device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_t cap,
+                                          const uint32_t *val, uint32_t *bins)
+{
+    uint32_t lane = lane_id();
+    for (uint32_t j = lane; j <= cap; j += KC_WAVE)
+        bins[j] = 0;
+    __threadfence_block();
+    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+    for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t ii = i + (uint64_t)k * KC_WAVE;
+            x[k] = ii < e ? val[a.col[ii]] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (x[k] != 0xffffffffu)
+                atomicAdd(&bins[x[k] < cap ? x[k] : cap], 1u);
+    }
+    __threadfence_block();
+}
+
+// h-index of v's neighbour values capped at cap, computed by one wave.
+template <typename R, int BLOCK>
+__device__ __forceinline__ uint32_t wave_hindex(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v,
+                                                uint32_t cap)
+{
+    if (cap == 0)
+        return 0;
+    uint32_t w = threadIdx.x / KC_WAVE;
+    uint32_t sum;
+    if (cap < KC_LBINS) {
+        wave_hist<R, true>(a, v, cap, a.core, L.bins[w]);
+        return wave_hsearch(L.bins[w], cap, sum);
+    }
+    uint32_t *gb = a.histo + hbase(a.rowptr, v);
+    wave_hist<R, false>(a, v, cap, a.core, gb);
+    return wave_hsearch(gb, cap, sum);
+}
+
+// ----------------------------------------------------- block-level paths
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_sum(HLds<BLOCK> &L, uint32_t x)
+{
+    constexpr int NW = BLOCK / KC_WAVE;
+    x = wave_sum_u32(x);
+    __syncthreads();
+    if (lane_id() == 0)
+        L.red[threadIdx.x / KC_WAVE] = x;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int i = 0; i < NW; i++)
+        t += L.red[i];
+    __syncthreads();
+    return t;
+}
+
+// Block-wide top-down search over HBM bins[1..top].
+template <int BLOCK>
+__device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t top,
+                                  uint32_t &sum)
+{
+    constexpr int NW = BLOCK / KC_WAVE;
+    uint32_t carry = 0;
+    uint32_t w = threadIdx.x / KC_WAVE;
+    for (int64_t hi = top; hi >= 1; hi -= BLOCK) {
+        int64_t j = hi - threadIdx.x;
+        bool valid = j >= 1;
+        uint32_t b = valid ? ld_relaxed(&bins[j]) : 0;
+        uint32_t incl = wave_incl_scan_u32(b);
+        __syncthreads();
+        if (lane_id() == KC_WAVE - 1)
+            L.red[w] = incl;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (int i = 0; i < NW; i++) {
+            uint32_t t = L.red[i];
+            if (i < (int)w)
+                before += t;
+            tot += t;
+        }
+        uint32_t s = carry + before + incl;
+        bool ok = valid && s >= (uint32_t)j;
+        if (threadIdx.x == 0)
+            L.bk = 0xffffffffu;
+        __syncthreads();
+        if (ok)
+            atomicMin(&L.bk, threadIdx.x);
+        __syncthreads();
+        uint32_t f = L.bk;
+        if (f != 0xffffffffu) {
+            if (threadIdx.x == f)
+                L.bsum = s;
+            __syncthreads();
+            sum = L.bsum;
+            return (uint32_t)(hi - f);
+        }
+        carry += tot;
+    }
+    sum = carry;
+    return 0;
+}
+
+// Block-wide histogram of min(cap, val(u)) into HBM bins[0..cap].
+template <typename R, int BLOCK>
+__device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t *bins)
+{
+    for (uint32_t j = threadIdx.x; j <= cap; j += BLOCK)
+        bins[j] = 0;
+    __threadfence();
+    __syncthreads();
+    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+    for (uint64_t i = s + threadIdx.x; i < e; i += BLOCK) {
+        uint32_t x = a.core[a.col[i]];
+        atomicAdd(&bins[x < cap ? x : cap], 1u);
+    }
+    __threadfence();
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ grab loop
+// Runs f(item) for items [0, n) of a list (or the vertex range when lst is
+// null), one item per wave; items with more than KC_HBIG neighbours are
+// queued for the whole workgroup and run with fb(item) afterwards.
+template <int BLOCK, typename R, typename F, typename FB>
+__device__ void for_items(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, uint32_t n,
+                          const uint32_t *lst, F &&f, FB &&fb)
+{
+    uint32_t lane = lane_id();
+    GridSync *gs = a.gs;
+    if (threadIdx.x == 0)
+        L.nbig = 0;
+    __syncthreads();
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0)
+            base = atomicAdd(&gs->slot[phase & 1].qhead, KC_HGRAB);
+        base = __shfl(base, 0, KC_WAVE);
+        if (base >= n)
+            break;
+        uint32_t end = base + KC_HGRAB < n ? base + KC_HGRAB : n;
+        for (uint32_t i = base; i < end; i++) {
+            uint32_t v = lst ? lst[i] : i;
+            uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+            if (d > KC_HBIG) {
+                uint32_t slot = 0;
+                if (lane == 0)
+                    slot = atomicAdd(&L.nbig, 1u);
+                slot = __shfl(slot, 0, KC_WAVE);
+                if (slot < KC_BIGQ) {
+                    if (lane == 0)
+                        L.bigq[slot] = v;
+                    continue;
+                }
+            }
+            f(v, d);
+        }
+    }
+    __syncthreads();
+    uint32_t nbig = L.nbig < KC_BIGQ ? L.nbig : KC_BIGQ;
+    for (uint32_t i = 0; i < nbig; i++)
+        fb(L.bigq[i]);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- kernel
+template <int BLOCK, typename R>
+__global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
+{
+    __shared__ HLds<BLOCK> L;
+    GridSync *gs = a.gs;
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / KC_WAVE;
+    const uint32_t V = a.V;
+    uint32_t gen = 0;
+    uint32_t round = 0;
+    uint64_t changed_total = 0;
+    uint64_t upd_edges = 0; // adjacency entries of changed vertices walked (HistoCore)
+    uint32_t cap_list = V ? V : 1;
+    uint32_t *act_cur = a.list[0]; // CntCore active lists, swapped per round
+    uint32_t *act_nxt = a.list[3];
+    uint32_t n_act_prev = 0;
+
+    // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
+    //      (histo_kernel, HistoCore.cu:15-75)
+    if (a.method == HM_HISTO) {
+        uint32_t phase = gen;
+        for_items<BLOCK, R>(
+            L, a, phase, V, nullptr,
+            [&](uint32_t v, uint32_t d) {
+                if (d == 0)
+                    return;
+                uint64_t b = hbase(a.rowptr, v);
+                if (d < KC_LBINS) {
+                    wave_hist<R, true>(a, v, d, a.core, L.bins[wv]);
+                    for (uint32_t j = lane; j <= d; j += KC_WAVE)
+                        a.histo[b + j] = j == 0 ? d : L.bins[wv][j];
+                } else {
+                    wave_hist<R, false>(a, v, d, a.core, a.histo + b);
+                    if (lane == 0)
+                        a.histo[b] = d;
+                }
+            },
+            [&](uint32_t v) {
+                uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+                uint64_t b = hbase(a.rowptr, v);
+                block_hist<R, BLOCK>(a, v, d, a.histo + b);
+                if (threadIdx.x == 0)
+                    a.histo[b] = d;
+                __syncthreads();
+            });
+        if (!grid_barrier(gs, gen))
+            goto out;
+    }
+
+    for (;;) {
+        round++;
+        // ------------------------------------------------ count phase
+        uint32_t phase = gen;
+        uint32_t n_in = 0;
+        const uint32_t *in_list = nullptr;
+        uint32_t *c_list = a.list[1];
+        if (round == 1) {
+            n_in = V; // every vertex; the per-method filter is applied below
+        } else {
+            n_in = L.done; // length of the list the previous phase built
+            in_list = a.method == HM_CNT ? act_cur : a.list[0];
+        }
+        if (a.method == HM_HISTO) {
+            // count_kernel (HistoCore.cu:77-128): scan bins from the old core
+            // down; collapse on change
+            auto one = [&](uint32_t v, uint32_t d) {
+                uint32_t old = a.core[v];
+                if (round == 1 && old <= 1)
+                    return;
+                uint64_t b = hbase(a.rowptr, v);
+                uint32_t sum;
+                uint32_t k = wave_hsearch(a.histo + b + 0, old, sum);
+                bool ch = k != old;
+                if (ch && lane == 0) {
+                    a.histo[b] = old;
+                    a.histo[b + k] = sum;
+                    a.core[v] = k;
+                }
+                hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+            };
+            for_items<BLOCK, R>(L, a, phase, n_in, in_list, one,
+                                [&](uint32_t v) {
+                                    uint32_t old = a.core[v];
+                                    if (round == 1 && old <= 1)
+                                        return;
+                                    uint64_t b = hbase(a.rowptr, v);
+                                    uint32_t sum;
+                                    uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
+                                    bool ch = k != old;
+                                    if (ch && threadIdx.x == 0) {
+                                        a.histo[b] = old;
+                                        a.histo[b + k] = sum;
+                                        a.core[v] = k;
+                                    }
+                                    if (wv == 0)
+                                        hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                                    __syncthreads();
+                                });
+        } else {
+            // NbrCore count_kernel (NbrCore.cu:78-113) / CntCore count_kernel
+            // (CntCore.cu:82-130): recompute from scratch; the new value is
+            // deferred to newc[] so every vertex of this round reads the
+            // previous round's values
+            auto want = [&](uint32_t v) -> bool {
+                if (a.method == HM_CNT)
+                    return round == 1 ? a.core[v] > 0 : true; // list holds cnt<core
+                return true;
+            };
+            for_items<BLOCK, R>(
+                L, a, phase, n_in, in_list,
+                [&](uint32_t v, uint32_t d) {
+                    if (!want(v))
+                        return;
+                    uint32_t old = a.core[v];
+                    uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old);
+                    bool ch = k != old;
+                    if (ch && lane == 0)
+                        a.newc[v] = k;
+                    hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                },
+                [&](uint32_t v) {
+                    if (!want(v))
+                        return;
+                    uint32_t old = a.core[v];
+                    uint32_t sum;
+                    uint64_t b = hbase(a.rowptr, v);
+                    block_hist<R, BLOCK>(a, v, old, a.histo + b);
+                    uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
+                    bool ch = k != old;
+                    if (ch && threadIdx.x == 0)
+                        a.newc[v] = k;
+                    if (wv == 0)
+                        hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                    __syncthreads();
+                });
+        }
+        if (!grid_barrier(gs, gen))
+            goto out;
+        uint32_t n_changed = ld_relaxed(&gs->slot[phase & 1].qtail);
+        changed_total += n_changed;
+        if (n_changed == 0)
+            break;
+
+        // ------------------------------------------------ update phase
+        phase = gen;
+        if (a.method == HM_HISTO) {
+            // update_histo_kernel (HistoCore.cu:130-202)
+            uint32_t *f_list = a.list[0];
+            for_items<BLOCK, R>(
+                L, a, phase, n_changed, c_list,
+                [&](uint32_t v, uint32_t d) {
+                    // lanes over the neighbours; the trip count is uniform
+                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    uint64_t b = hbase(a.rowptr, v);
+                    uint32_t nc = a.core[v], oc = a.histo[b];
+                    if (lane == 0)
+                        upd_edges += e - s;
+                    for (uint64_t i0 = s; i0 < e; i0 += KC_WAVE) {
+                        uint64_t i = i0 + lane;
+                        bool flag = false;
+                        uint32_t u = 0;
+                        if (i < e) {
+                            u = a.col[i];
+                            uint32_t cu = a.core[u];
+                            if (nc < cu) {
+                                uint64_t bu = hbase(a.rowptr, u);
+                                uint32_t sub = cu < oc ? cu : oc;
+                                uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
+                                atomicAdd(&a.histo[bu + nc], 1u);
+                                flag = oc >= cu && c == cu;
+                            }
+                        }
+                        hpush(gs, phase, f_list, cap_list, flag, u);
+                    }
+                },
+                [&](uint32_t v) {
+                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    uint64_t b = hbase(a.rowptr, v);
+                    uint32_t nc = a.core[v], oc = a.histo[b];
+                    if (threadIdx.x == 0)
+                        upd_edges += e - s;
+                    for (uint64_t i0 = s; i0 < e; i0 += BLOCK) {
+                        uint64_t i = i0 + threadIdx.x;
+                        bool flag = false;
+                        uint32_t u = 0;
+                        if (i < e) {
+                            u = a.col[i];
+                            uint32_t cu = a.core[u];
+                            if (nc < cu) {
+                                uint64_t bu = hbase(a.rowptr, u);
+                                uint32_t sub = cu < oc ? cu : oc;
+                                uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
+                                atomicAdd(&a.histo[bu + nc], 1u);
+                                flag = oc >= cu && c == cu;
+                            }
+                        }
+                        hpush(gs, phase, f_list, cap_list, flag, u);
+                    }
+                    __syncthreads();
+                });
+            if (!grid_barrier(gs, gen))
+                goto out;
+            if (threadIdx.x == 0)
+                L.done = ld_relaxed(&gs->slot[phase & 1].qtail);
+            __syncthreads();
+            continue;
+        }
+
+        // NbrCore / CntCore: apply the deferred values and mark neighbours
+        // (nbr_kernel, NbrCore.cu:115-167 / CntCore.cu:132-195)
+        {
+            uint32_t *m_list = a.method == HM_NBR ? a.list[0] : a.list[2];
+            uint32_t epoch = round;
+            for_items<BLOCK, R>(
+                L, a, phase, n_changed, c_list,
+                [&](uint32_t v, uint32_t d) {
+                    if (lane == 0)
+                        a.core[v] = a.newc[v];
+                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    for (uint64_t i0 = s; i0 < e; i0 += KC_WAVE) {
+                        uint64_t i = i0 + lane;
+                        bool fresh = false;
+                        uint32_t u = 0;
+                        if (i < e) {
+                            u = a.col[i];
+                            fresh = atomicMax(&a.mark[u], epoch) != epoch;
+                            if (a.method == HM_CNT)
+                                a.cnt[u] = 0;
+                        }
+                        hpush(gs, phase, m_list, cap_list, fresh, u);
+                    }
+                },
+                [&](uint32_t v) {
+                    if (threadIdx.x == 0)
+                        a.core[v] = a.newc[v];
+                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    for (uint64_t i0 = s; i0 < e; i0 += BLOCK) {
+                        uint64_t i = i0 + threadIdx.x;
+                        bool fresh = false;
+                        uint32_t u = 0;
+                        if (i < e) {
+                            u = a.col[i];
+                            fresh = atomicMax(&a.mark[u], epoch) != epoch;
+                            if (a.method == HM_CNT)
+                                a.cnt[u] = 0;
+                        }
+                        hpush(gs, phase, m_list, cap_list, fresh, u);
+                    }
+                    __syncthreads();
+                });
+            if (!grid_barrier(gs, gen))
+                goto out;
+            uint32_t n_marked = ld_relaxed(&gs->slot[phase & 1].qtail);
+            if (a.method == HM_NBR) {
+                if (threadIdx.x == 0)
+                    L.done = n_marked;
+                __syncthreads();
+                continue;
+            }
+
+            // CntCore: recount around changed vertices (cnt_kernel,
+            // CntCore.cu:197-264) and rebuild the active list
+            // {v : cnt[v] < core[v]} from the old active list and the
+            // recounted vertices
+            phase = gen;
+            uint32_t *act_in = act_cur;
+            uint32_t *act_out = act_nxt;
+            uint32_t n_act_in = round == 1 ? V : n_act_prev;
+            for_items<BLOCK, R>(
+                L, a, phase, n_marked, a.list[2],
+                [&](uint32_t u, uint32_t d) {
+                    uint32_t cu = a.core[u];
+                    uint32_t c = 0;
+                    uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+                    for (uint64_t i = s + lane; i < e; i += KC_WAVE)
+                        c += a.core[a.col[i]] >= cu;
+                    c = wave_sum_u32(c);
+                    if (lane == 0)
+                        a.cnt[u] = c;
+                    hpush(gs, phase, act_out, cap_list, lane == 0 && c < cu, u);
+                },
+                [&](uint32_t u) {
+                    uint32_t cu = a.core[u];
+                    uint32_t c = 0;
+                    uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+                    for (uint64_t i = s + threadIdx.x; i < e; i += BLOCK)
+                        c += a.core[a.col[i]] >= cu;
+                    c = block_sum<BLOCK>(L, c);
+                    if (threadIdx.x == 0)
+                        a.cnt[u] = c;
+                    if (wv == 0)
+                        hpush(gs, phase, act_out, cap_list, lane == 0 && c < cu, u);
+                    __syncthreads();
+                });
+            // members of the old active set that were not recounted keep
+            // their cnt; they stay active while cnt < core.  Round 1's
+            // active set is every vertex with degree > 0.
+            {
+                uint32_t nwaves = gridDim.x * (BLOCK / KC_WAVE);
+                uint32_t gw = blockIdx.x * (BLOCK / KC_WAVE) + wv;
+                for (uint32_t i0 = gw * KC_WAVE; i0 < n_act_in; i0 += nwaves * KC_WAVE) {
+                    uint32_t i = i0 + lane;
+                    bool keep = false;
+                    uint32_t v = 0;
+                    if (i < n_act_in) {
+                        v = round == 1 ? i : act_in[i];
+                        bool was_active = round == 1 ? (a.rowptr[v + 1] > a.rowptr[v]) : true;
+                        keep = was_active && a.mark[v] != epoch && a.cnt[v] < a.core[v];
+                    }
+                    hpush(gs, phase, act_out, cap_list, keep, v);
+                }
+            }
+            if (!grid_barrier(gs, gen))
+                goto out;
+            uint32_t n_act = ld_relaxed(&gs->slot[phase & 1].qtail);
+            act_cur = act_out;
+            act_nxt = act_in;
+            n_act_prev = n_act;
+            if (threadIdx.x == 0)
+                L.done = n_act;
+            __syncthreads();
+        }
+    }
+out:
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.stats[KC_ST_ITERS] = round;
+        a.stats[KC_ST_PHASES] = gen;
+        a.stats[KC_ST_CHANGED] = changed_total;
+        a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
+    }
+    uint64_t ue = wave_sum_u64(upd_edges);
+    if (lane == 0 && ue)
+        atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], (unsigned long long)ue);
+}
+
+template <typename R>
+__global__ void hindex_init_kernel(const R *rowptr, uint32_t V, uint32_t *core, uint32_t *cnt,
+                                   uint32_t *mark, GridSync *gs, uint32_t nblocks,
+                                   uint64_t timeout, uint64_t *stats)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = i; v < V; v += stride) {
+        core[v] = (uint32_t)(rowptr[v + 1] - rowptr[v]);
+        if (cnt)
+            cnt[v] = 0;
+        if (mark)
+            mark[v] = 0;
+    }
+    if (i == 0) {
+        gs->arrive = 0;
+        gs->gen = 0;
+        gs->err = 0;
+        gs->nblocks = nblocks;
+        gs->timeout_ticks = timeout;
+        gs->t0 = 0;
+        for (int p = 0; p < 2; p++) {
+            gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
+            gs->slot[p].min0 = 0xffffffffu;
+            gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
+        }
+        for (int k = 0; k < KC_ST_COUNT; k++)
+            stats[k] = 0;
+    }
+}
+
+template <int BLOCK, typename R>
+static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
+{
+    HArgs<R> a;
+    a.rowptr = (const R *)p.rowptr;
+    a.col = p.col;
+    a.V = p.V;
+    a.method = p.method;
+    a.core = p.core;
+    a.histo = p.histo;
+    a.cnt = p.cnt;
+    a.newc = p.newc;
+    a.mark = p.mark;
+    a.list[0] = p.list0;
+    a.list[1] = p.list1;
+    a.list[2] = p.list2;
+    a.list[3] = p.list3;
+    a.gs = p.gs;
+    a.stats = p.stats;
+    hipLaunchKernelGGL((hindex_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V,
+                       p.core, p.method == HM_CNT ? p.cnt : nullptr,
+                       p.method != HM_HISTO ? p.mark : nullptr, p.gs, p.grid, p.timeout_ticks,
+                       p.stats);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    void *args[] = {&a};
+    e = hipLaunchCooperativeKernel((const void *)hindex_kernel<BLOCK, R>, dim3(p.grid),
+                                   dim3(BLOCK), args, 0, st);
+    return e;
+}
+
+template <int BLOCK> static int hocc_t(bool wide)
+{
+    int nb = 0;
+    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &nb, hindex_kernel<BLOCK, uint64_t>, BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                              &nb, hindex_kernel<BLOCK, uint32_t>, BLOCK, 0);
+    return e == hipSuccess ? nb : 0;
+}
+
+int hindex_occupancy(uint32_t block, bool wide)
+{
+    switch (block) {
+    case 256: return hocc_t<256>(wide);
+    case 512: return hocc_t<512>(wide);
+    case 1024: return hocc_t<1024>(wide);
+    default: return 0;
+    }
+}
+
+hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st)
+{
+    switch (p.block) {
+    case 256: return p.wide ? launch_hindex_t<256, uint64_t>(p, st) : launch_hindex_t<256, uint32_t>(p, st);
+    case 512: return p.wide ? launch_hindex_t<512, uint64_t>(p, st) : launch_hindex_t<512, uint32_t>(p, st);
+    case 1024: return p.wide ? launch_hindex_t<1024, uint64_t>(p, st) : launch_hindex_t<1024, uint32_t>(p, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace kc

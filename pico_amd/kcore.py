@@ -46,7 +46,8 @@ class Stats(ctypes.Structure):
                 ("edges_walked", ctypes.c_uint64), ("kmax", ctypes.c_uint32),
                 ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
                 ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 4)]
+                ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 3)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_
@@ -361,3 +362,15 @@ def device_trace(dgraph, max_phases=1 << 16):
         "max_rounds": buf[:n, 8].copy(), "edges": buf[:n, 9].copy(),
         "wg0_rounds": buf[:n, 10].copy(), "max_wg_edges": buf[:n, 11].copy(),
     }
+
+
+def device_histogram(dgraph, nwords):
+    """HistoCore's final histogram (reference layout, nnz+V words)."""
+    L = lib()
+    if not getattr(L.kc_graph_histogram, "_kc_sig", False):
+        L.kc_graph_histogram.argtypes = [_vp, _vp, ctypes.c_uint64]
+        L.kc_graph_histogram.restype = ctypes.c_int
+        L.kc_graph_histogram._kc_sig = True
+    out = np.zeros(max(nwords, 1), np.uint32)
+    _check(L.kc_graph_histogram(dgraph._h, out.ctypes.data_as(_vp), nwords))
+    return out[:nwords]

@@ -1,0 +1,105 @@
+"""GPU parity of the h-index family (HistoCore, CntCore, NbrCore) against the
+oracle, through the C ABI: core numbers bit-exact vs Batagelj-Zaversnik, the
+"Number of iterations" equal to the oracle's restatement of each reference
+host loop (HistoCore.cu:301-341, CntCore.cu:344-417, NbrCore.cu:244-297) and
+to the plain Jacobi round count, and HistoCore's final histogram equal word
+for word to the restatement's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pico_amd as pa
+from pico_amd.kcore import device_histogram
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+HIDX = {"histocore": oracle.histocore, "cntcore": oracle.cntcore, "nbrcore": oracle.nbrcore}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pa.Context(0)
+    yield c
+    c.close()
+
+
+def _check(ctx, rowptr, col, method, width=4, hist=False, **opts):
+    hg = pa.HostGraph.from_csr(rowptr, col)
+    dg = ctx.upload(hg, rowptr_bytes=width)
+    try:
+        core, st = dg.decompose(method, **opts)
+        if hist:
+            dev_hist = device_histogram(dg, hg.nnz + hg.V)
+    finally:
+        pass
+    want = oracle.bz(rowptr, col)
+    np.testing.assert_array_equal(core, want)
+    if method == "histocore":
+        ref_core, it, ref_hist = oracle.histocore(rowptr, col, want_hist=True)
+        if hist:
+            np.testing.assert_array_equal(dev_hist, ref_hist)
+    else:
+        ref_core, it = HIDX[method](rowptr, col)
+    _, jac = oracle.hindex_jacobi(rowptr, col)
+    assert st["iterations"] == it == jac, (st["iterations"], it, jac)
+    dg.close()
+    return core, st
+
+
+METHODS = ["histocore", "cntcore", "nbrcore"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_example_txt(ctx, method):
+    rp, col = graphs.example()
+    core, _ = _check(ctx, rp, col, method, hist=True)
+    assert core.tolist() == [2, 2, 2, 2, 2]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("name", graphs.CLOSED_FORMS)
+def test_closed_forms(ctx, method, name):
+    rp, col, expect = graphs.closed_form(name)
+    core, _ = _check(ctx, rp, col, method)
+    np.testing.assert_array_equal(core, expect)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_er_config(ctx, method):
+    hg = pa.HostGraph.er(10000, 50000, seed=0)
+    _check(ctx, hg.rowptr, hg.col, method, hist=True)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("scale", [10, 14])
+def test_rmat_small(ctx, method, scale):
+    hg = pa.HostGraph.rmat(scale, 16, seed=scale)
+    _check(ctx, hg.rowptr, hg.col, method, hist=(method == "histocore"))
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_chunglu_hubs_wide_rowptr(ctx, method):
+    # hubs above 4096 neighbours exercise the workgroup-level paths
+    hg = pa.HostGraph.chunglu(1 << 15, beta=2.1, mean_deg=20.0, cap=20000.0, seed=5)
+    _check(ctx, hg.rowptr, hg.col, method, width=8, hist=(method == "histocore"))
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_launch_shapes(ctx, method, block):
+    hg = pa.HostGraph.rmat(12, 16, seed=7)
+    _check(ctx, hg.rowptr, hg.col, method, block=block)
+    _check(ctx, hg.rowptr, hg.col, method, block=block, grid=3)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_multigraph_from_coo_reader(ctx, method, tmp_path):
+    # the COO reader keeps duplicate edges (shared/graph.cpp:49-81): cores of
+    # the multigraph, as every reference method computes them
+    p = tmp_path / "dup.txt"
+    p.write_text("%%header\n5 5 9\n1 2 1\n2 1 1\n1 3 1\n2 3 1\n3 4 1\n4 5 1\n5 3 1\n4 5 1\n2 2 1\n")
+    hg = pa.HostGraph.read_coo(str(p))
+    assert hg.multigraph and hg.edges_read == 8
+    _check(ctx, hg.rowptr, hg.col, method)
