@@ -67,7 +67,7 @@ typedef struct kc_opts {
     uint32_t queue_cap;  /* shared work-list capacity in items */
     uint32_t gpp_compat; /* KC_GPP: report isolated vertices as 1 (reference quirk) */
     uint32_t part_size;  /* PeelOne reference part width (reporting only) */
-    uint32_t local_cap;  /* soft cap of a workgroup's local frontier; the
+    uint32_t local_cap;  /* pending-edge cap of a workgroup's local frontier; the
                             excess is shared with every workgroup */
     uint32_t flags;      /* KC_OPT_* */
     uint32_t reserved[4];

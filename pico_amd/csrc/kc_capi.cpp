@@ -526,7 +526,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.col = g->col;
     p.V = g->V;
     p.hub_split = hub;
-    p.local_cap = o.local_cap ? o.local_cap : 1024;
+    p.local_cap = o.local_cap; // 0: 16 pending edges per thread
     p.deg = (uint32_t *)g->deg.p;
     p.done = (uint64_t *)g->done.p;
     p.list0 = (uint64_t *)g->list0.p;

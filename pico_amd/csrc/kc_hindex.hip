@@ -307,9 +307,12 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         for_items<BLOCK, R>(
             L, a, phase, V, nullptr,
             [&](uint32_t v, uint32_t d) {
-                if (d == 0)
-                    return;
                 uint64_t b = hbase(a.rowptr, v);
+                if (d == 0) { // the reference zeroes the whole array first (:286)
+                    if (lane == 0)
+                        a.histo[b] = 0;
+                    return;
+                }
                 if (d < KC_LBINS) {
                     wave_hist<R, true>(a, v, d, a.core, L.bins[wv]);
                     for (uint32_t j = lane; j <= d; j += KC_WAVE)

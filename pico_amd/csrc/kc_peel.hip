@@ -31,7 +31,7 @@ template <typename R> struct PeelArgs {
     const uint32_t *col;
     uint32_t V;
     uint32_t hub_split;
-    uint32_t local_cap; // soft cap of the local frontier (<= LQ); excess spills
+    uint32_t local_cap; // pending-edge cap of the local frontier; the excess spills
     uint32_t *deg;
     uint64_t *done;
     uint64_t *list0;
@@ -52,6 +52,7 @@ constexpr uint32_t KC_DEAD = 0xffffffffu;
 template <int BLOCK, int LQ> struct PeelLds {
     uint32_t lbuf[2][LQ]; // local frontier, double buffered
     uint32_t cnt[2];      // push counters (may exceed LQ: the excess spilled)
+    uint32_t pend[2];     // edges pending in each local buffer
     uint64_t start[BLOCK];
     uint64_t pre[BLOCK + 1];
     uint32_t chunks[KC_MAXC]; // this WG's chunks that may still hold live vertices
@@ -130,22 +131,31 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
         atomicAdd(&L.claims, (uint32_t)__popcll(cm));
     uint32_t d0 = claim ? deg0(a.rowptr, u) : 0;
     bool hub = claim && d0 > a.hub_split;
-    bool loc = claim && !hub;
+    bool loc = claim && !hub && d0 > 0; // a degree-0 claim has nothing to walk
     uint64_t lm = ballot(loc);
-    uint32_t idx = 0xffffffffu;
+    bool spill = false;
     if (lm) {
+        // The local frontier holds at most local_cap pending edges; once the
+        // WG's backlog is that large, further claims go to the shared list so
+        // a wide cascade is spread over every workgroup.
+        uint32_t wsum = wave_sum_u32(loc ? d0 : 0u);
         uint32_t leader = __ffsll((unsigned long long)lm) - 1;
-        uint32_t base = 0;
-        if (lane == leader)
-            base = atomicAdd(&L.cnt[buf], (uint32_t)__popcll(lm));
+        uint32_t base = 0xffffffffu;
+        if (lane == leader) {
+            uint32_t old_e = atomicAdd(&L.pend[buf], wsum);
+            if (old_e < a.local_cap)
+                base = atomicAdd(&L.cnt[buf], (uint32_t)__popcll(lm));
+        }
         base = __shfl(base, leader, KC_WAVE);
         if (loc) {
-            idx = base + lane_rank(lm);
-            if (idx < a.local_cap)
+            uint32_t idx = base == 0xffffffffu ? 0xffffffffu : base + lane_rank(lm);
+            if (idx < (uint32_t)LQ)
                 L.lbuf[buf][idx] = u;
+            else
+                spill = true;
         }
     }
-    push_shared(a, phase, loc && idx >= a.local_cap, (uint64_t)u);
+    push_shared(a, phase, spill, (uint64_t)u);
     if (hub) {
         PhaseSlot *slot = &a.gs->slot[phase & 1];
         uint64_t *list = (phase & 1) ? a.list1 : a.list0;
@@ -277,8 +287,8 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
     for (;;) {
         __syncthreads();
         uint32_t n = L.cnt[cur];
-        if (n > a.local_cap)
-            n = a.local_cap;
+        if (n > (uint32_t)LQ)
+            n = LQ;
         if (n == 0)
             break;
         for (uint32_t base = 0; base < n; base += BLOCK) {
@@ -297,6 +307,7 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
         __syncthreads();
         if (threadIdx.x == 0) {
             L.cnt[cur] = 0;
+            L.pend[cur] = 0;
             L.rounds++;
         }
         cur ^= 1;
@@ -517,6 +528,8 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     if (threadIdx.x == 0) {
         L.cnt[0] = 0;
         L.cnt[1] = 0;
+        L.pend[0] = 0;
+        L.pend[1] = 0;
     }
     for (;;) {
         if (threadIdx.x == 0) {
@@ -714,7 +727,7 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.col = p.col;
     a.V = p.V;
     a.hub_split = p.hub_split;
-    a.local_cap = p.local_cap < (uint32_t)LQ && p.local_cap ? p.local_cap : (uint32_t)LQ;
+    a.local_cap = p.local_cap ? p.local_cap : (uint32_t)BLOCK * 16;
     a.deg = p.deg;
     a.done = p.done;
     a.list0 = p.list0;
