@@ -57,7 +57,8 @@ typedef enum kc_method {
 } kc_method;
 
 /* kc_opts.flags bits */
-#define KC_OPT_TRACE 1u /* record a per-phase device trace (kc_graph_trace) */
+#define KC_OPT_TRACE 1u       /* record a per-phase device trace (kc_graph_trace) */
+#define KC_OPT_NO_DEGSORT 2u  /* HistoCore: walk the adjacency in input order */
 
 /* Launch options; zero fields select defaults (or the autotuner's pick). */
 typedef struct kc_opts {
@@ -87,7 +88,9 @@ typedef struct kc_stats {
     double device_ms; /* HIP-event time of the decomposition on its stream */
     double total_ms;  /* host wall time of the call incl. copies */
     uint64_t changed; /* h-index: core-value changes over all rounds */
-    uint64_t reserved[3];
+    double prep_ms;   /* one-time layout preparation done by this call (HistoCore's
+                         degree-ordered adjacency), outside device_ms */
+    uint64_t reserved[2];
 } kc_stats;
 
 KC_API const char *kc_last_error(void);

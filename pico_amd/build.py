@@ -29,6 +29,7 @@ HIP_FLAGS = [
 LIB_SOURCES = [
     os.path.join(CSRC, "kc_peel.hip"),
     os.path.join(CSRC, "kc_hindex.hip"),
+    os.path.join(CSRC, "kc_prep.hip"),
     os.path.join(CSRC, "kc_capi.cpp"),
     os.path.join(HOST, "kc_host.cpp"),
 ]

@@ -78,6 +78,8 @@ struct kc_graph {
     uint64_t bytes = 0;
     // method scratch, allocated on first use and kept for reuse
     DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
+    DevBuf col_ds, nd_ds; // degree-ordered adjacency (built on first HistoCore run)
+    bool degsorted = false;
     uint32_t trace_phases = 0;
     const uint32_t *result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -429,7 +431,7 @@ void kc_graph_free(kc_graph *g)
     hipFree(g->col);
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
-                      &g->trace})
+                      &g->trace, &g->col_ds, &g->nd_ds})
         if (b->p)
             hipFree(b->p);
     if (g->ev0)
@@ -618,6 +620,32 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     if (method == KC_CNT &&
         ((s = ensure(g, g->cnt, V4, "cnt")) || (s = ensure(g, g->list0, V4, "list"))))
         return s;
+    // HistoCore queues changed vertices as 64-edge virtual nodes
+    uint64_t chunk_cap = (uint64_t)g->V + g->nnz / 64 + 64;
+    if (chunk_cap > 0xffffffffull)
+        chunk_cap = 0xffffffffull;
+    if (method == KC_HISTO && (s = ensure(g, g->list1, (size_t)chunk_cap * 8, "virtual nodes")))
+        return s;
+    // degree-ordered adjacency for HistoCore's update rounds, built once
+    bool use_ds = method == KC_HISTO && !(o.flags & KC_OPT_NO_DEGSORT) && g->nnz &&
+                  g->nnz <= 0x7fffffffull;
+    if (use_ds && !g->degsorted) {
+        size_t nb = (size_t)g->nnz * 4;
+        if ((s = ensure(g, g->col_ds, nb, "ordered adjacency")) ||
+            (s = ensure(g, g->nd_ds, nb, "neighbour degrees")))
+            return s;
+        double tp = now_ms();
+        // the histogram buffer (>= nnz words) is free scratch before the run
+        hipError_t pe = kc::build_degree_sorted(g->rowptr, g->wide, g->col, g->V, g->nnz,
+                                                (uint32_t *)g->col_ds.p, (uint32_t *)g->nd_ds.p,
+                                                (uint32_t *)g->histo.p, c->stream);
+        if (pe == hipSuccess)
+            pe = hipStreamSynchronize(c->stream);
+        if (pe != hipSuccess)
+            return hip_fail(pe, "degree-ordered adjacency");
+        g->degsorted = true;
+        st.prep_ms = now_ms() - tp;
+    }
     kc::HindexLaunch p{};
     p.rowptr = g->rowptr;
     p.col = g->col;
@@ -632,12 +660,24 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.list1 = (uint32_t *)g->hl1.p;
     p.list2 = (uint32_t *)g->hl2.p;
     p.list3 = (uint32_t *)g->list0.p;
+    p.chunks = (uint64_t *)g->list1.p;
+    p.chunk_cap = (uint32_t)chunk_cap;
+    p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
+    p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;
     p.grid = grid;
     p.block = block;
     p.wide = g->wide;
     p.timeout_ticks = c->clock_khz * 1000ull * 20ull;
+    g->trace_phases = 0;
+    if (o.flags & KC_OPT_TRACE) {
+        if ((s = ensure(g, g->trace, (size_t)KC_TRACE_CAP * kc::KC_TRACE_WORDS * 8, "trace")))
+            return s;
+        KC_HIP(hipMemsetAsync(g->trace.p, 0, g->trace.n, c->stream), "trace clear");
+        p.trace = (uint64_t *)g->trace.p;
+        p.trace_cap = KC_TRACE_CAP;
+    }
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_hindex(p, c->stream);
     if (e != hipSuccess)
@@ -667,6 +707,8 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     st.phases = sv[kc::KC_ST_PHASES];
     st.edges_walked = sv[kc::KC_ST_EDGES];
     st.changed = sv[kc::KC_ST_CHANGED];
+    if (p.trace)
+        g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     st.grid = grid;
     st.block = block;
     return KC_OK;

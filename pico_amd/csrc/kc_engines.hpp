@@ -66,16 +66,28 @@ struct HindexLaunch {
     uint32_t *newc;   // deferred new values, V words
     uint32_t *mark;   // epoch stamps, V words
     uint32_t *list0, *list1, *list2, *list3; // frontier / changed lists, V each
+    uint64_t *chunks; // HistoCore virtual nodes of changed vertices
+    uint32_t chunk_cap;
+    const uint32_t *col_ds; // degree-ordered adjacency, or null
+    const uint32_t *nd_ds;
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;
     bool wide;
     uint64_t timeout_ticks;
+    uint64_t *trace; // may be null
+    uint32_t trace_cap;
 };
 
 hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st);
 int hindex_occupancy(uint32_t block, bool wide);
 
 size_t grid_sync_bytes();
+
+// Rows re-ordered by neighbour degree, largest first (kc_prep.hip); nd_out
+// receives the neighbour degrees; nd_tmp is nnz words of scratch.
+hipError_t build_degree_sorted(const void *rowptr, bool wide, const uint32_t *col, uint32_t V,
+                               uint64_t nnz, uint32_t *col_out, uint32_t *nd_out,
+                               uint32_t *nd_tmp, hipStream_t st);
 
 } // namespace kc

@@ -35,6 +35,7 @@ constexpr uint32_t KC_LBINS = 512;  // LDS histogram bins per wavefront
 constexpr uint32_t KC_HBIG = 4096;  // items above this go to the workgroup
 constexpr uint32_t KC_HGRAB = 16;   // items taken per wavefront grab
 constexpr uint32_t KC_BIGQ = 256;   // per-WG queue of big items
+constexpr uint32_t KC_HCH = 64;     // edges per virtual node of a changed vertex
 
 template <typename R> struct HArgs {
     const R *rowptr;
@@ -47,8 +48,14 @@ template <typename R> struct HArgs {
     uint32_t *newc;  // deferred values (NbrCore / CntCore)
     uint32_t *mark;  // epoch stamps (NbrCore / CntCore)
     uint32_t *list[4];
+    uint64_t *chunks; // HistoCore: (vertex | chunk << 32) virtual nodes of changed vertices
+    uint32_t chunk_cap;
+    const uint32_t *col_ds; // degree-ordered adjacency (kc_prep.hip) or null
+    const uint32_t *nd_ds;  // neighbour degrees beside col_ds
     GridSync *gs;
     uint64_t *stats;
+    uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
+    uint32_t trace_cap;
 };
 
 template <int BLOCK> struct HLds {
@@ -57,7 +64,9 @@ template <int BLOCK> struct HLds {
     uint32_t nbig;
     uint32_t grab;
     uint32_t red[BLOCK / KC_WAVE];
-    uint32_t bk, bsum; // block-level h-index result
+    uint32_t bk, bsum, bq; // block-level h-index result
+    uint32_t stv[BLOCK / KC_WAVE][KC_HGRAB]; // per-wave staging of changed vertices
+    uint32_t stn[BLOCK / KC_WAVE][KC_HGRAB]; // and their virtual-node counts
     uint32_t done;
 };
 
@@ -111,6 +120,27 @@ __device__ __forceinline__ uint32_t wave_hsearch(const uint32_t *bins, uint32_t 
     }
     sum = carry;
     return 0;
+}
+
+// Number of entries > k at the head of nd[0..n), which is sorted descending:
+// a 64-ary search, one probe per lane per step.
+__device__ __forceinline__ uint64_t wave_count_gt(const uint32_t *nd, uint64_t n, uint32_t k)
+{
+    uint32_t lane = lane_id();
+    uint64_t lo = 0, hi = n; // entries below lo are > k, entries from hi on are <= k
+    while (hi - lo > KC_WAVE) {
+        uint64_t step = (hi - lo + KC_WAVE - 1) / KC_WAVE;
+        uint64_t p = lo + (uint64_t)lane * step;
+        uint32_t t = (uint32_t)__popcll(ballot(p < hi && nd[p] > k));
+        if (t == 0)
+            return lo;
+        uint64_t nlo = lo + (uint64_t)(t - 1) * step + 1;
+        uint64_t nhi = lo + (uint64_t)t * step;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+    }
+    uint64_t p = lo + lane;
+    return lo + (uint32_t)__popcll(ballot(p < hi && nd[p] > k));
 }
 
 // Histogram of min(cap, val(u)) over u in N(v) into bins[0..cap].  LDS or
@@ -174,19 +204,27 @@ __device__ __forceinline__ uint32_t block_sum(HLds<BLOCK> &L, uint32_t x)
     return t;
 }
 
-// Block-wide top-down search over HBM bins[1..top].
+// Block-wide top-down search over HBM bins[1..top]; each thread reads four
+// consecutive bins per step (4*BLOCK bins in flight per step).
 template <int BLOCK>
 __device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t top,
                                   uint32_t &sum)
 {
     constexpr int NW = BLOCK / KC_WAVE;
+    constexpr int Q = 4;
     uint32_t carry = 0;
-    uint32_t w = threadIdx.x / KC_WAVE;
-    for (int64_t hi = top; hi >= 1; hi -= BLOCK) {
-        int64_t j = hi - threadIdx.x;
-        bool valid = j >= 1;
-        uint32_t b = valid ? ld_relaxed(&bins[j]) : 0;
-        uint32_t incl = wave_incl_scan_u32(b);
+    const uint32_t w = threadIdx.x / KC_WAVE;
+    for (int64_t hi = top; hi >= 1; hi -= (int64_t)BLOCK * Q) {
+        int64_t j0 = hi - (int64_t)threadIdx.x * Q; // this thread: bins j0, j0-1, .., j0-3
+        uint32_t b[Q];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            int64_t j = j0 - q;
+            b[q] = j >= 1 ? bins[j] : 0u;
+            mine += b[q];
+        }
+        uint32_t incl = wave_incl_scan_u32(mine);
         __syncthreads();
         if (lane_id() == KC_WAVE - 1)
             L.red[w] = incl;
@@ -198,21 +236,35 @@ __device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t
                 before += t;
             tot += t;
         }
-        uint32_t s = carry + before + incl;
-        bool ok = valid && s >= (uint32_t)j;
+        uint32_t s = carry + before + incl - mine; // suffix sum above bin j0
+        int hitq = -1;
+        uint32_t hs = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            int64_t j = j0 - q;
+            s += b[q];
+            if (hitq < 0 && j >= 1 && s >= (uint32_t)j) {
+                hitq = q;
+                hs = s;
+            }
+        }
         if (threadIdx.x == 0)
             L.bk = 0xffffffffu;
         __syncthreads();
-        if (ok)
+        if (hitq >= 0)
             atomicMin(&L.bk, threadIdx.x);
         __syncthreads();
         uint32_t f = L.bk;
         if (f != 0xffffffffu) {
-            if (threadIdx.x == f)
-                L.bsum = s;
+            if (threadIdx.x == f) {
+                L.bsum = hs;
+                L.bq = (uint32_t)hitq; // offset of the hit within the thread
+            }
             __syncthreads();
             sum = L.bsum;
-            return (uint32_t)(hi - f);
+            uint32_t k = (uint32_t)(hi - (int64_t)f * Q - L.bq);
+            __syncthreads();
+            return k;
         }
         carry += tot;
     }
@@ -241,9 +293,9 @@ __device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t
 // Runs f(item) for items [0, n) of a list (or the vertex range when lst is
 // null), one item per wave; items with more than KC_HBIG neighbours are
 // queued for the whole workgroup and run with fb(item) afterwards.
-template <int BLOCK, typename R, typename F, typename FB>
-__device__ void for_items(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, uint32_t n,
-                          const uint32_t *lst, F &&f, FB &&fb)
+template <int BLOCK, typename R, typename F, typename FB, typename FE>
+__device__ void for_items_g(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, uint32_t n,
+                            const uint32_t *lst, F &&f, FB &&fb, FE &&fe)
 {
     uint32_t lane = lane_id();
     GridSync *gs = a.gs;
@@ -274,12 +326,53 @@ __device__ void for_items(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, uin
             }
             f(v, d);
         }
+        fe(); // end of this wave's grab
     }
     __syncthreads();
     uint32_t nbig = L.nbig < KC_BIGQ ? L.nbig : KC_BIGQ;
     for (uint32_t i = 0; i < nbig; i++)
         fb(L.bigq[i]);
     __syncthreads();
+}
+
+template <int BLOCK, typename R, typename F, typename FB>
+__device__ void for_items(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, uint32_t n,
+                          const uint32_t *lst, F &&f, FB &&fb)
+{
+    for_items_g<BLOCK, R>(L, a, phase, n, lst, f, fb, [] {});
+}
+
+// Grid barrier with the optional per-phase trace of kc_engines.hpp: latest
+// work-done time over the WGs, WG 0's exit time, round, items pushed in the
+// phase, and a phase kind (0 histogram build, 1 count, 2 update / apply,
+// 3 recount).
+template <typename R>
+__device__ __forceinline__ bool hbar(const HArgs<R> &a, GridSync *gs, uint32_t &gen,
+                                     uint32_t round, uint32_t kind)
+{
+    const uint32_t phase = gen;
+    uint64_t *tr = (a.trace && phase < a.trace_cap) ? a.trace + (uint64_t)phase * KC_TRACE_WORDS
+                                                    : nullptr;
+    if (tr) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = now_ticks();
+            atomicMax((unsigned long long *)&tr[3], (unsigned long long)t);
+            if (blockIdx.x == 0)
+                tr[1] = t;
+        }
+    }
+    bool ok = grid_barrier(gs, gen);
+    if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint64_t t = now_ticks();
+        tr[2] = t;
+        tr[4] = round;
+        tr[5] = ld_relaxed(&gs->slot[phase & 1].qtail);
+        tr[7] = kind;
+        if (phase + 1 < a.trace_cap)
+            a.trace[(uint64_t)(phase + 1) * KC_TRACE_WORDS] = t;
+    }
+    return ok;
 }
 
 // ---------------------------------------------------------------- kernel
@@ -299,6 +392,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     uint32_t *act_cur = a.list[0]; // CntCore active lists, swapped per round
     uint32_t *act_nxt = a.list[3];
     uint32_t n_act_prev = 0;
+    if (a.trace && blockIdx.x == 0 && threadIdx.x == 0)
+        a.trace[0] = now_ticks();
 
     // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
     //      (histo_kernel, HistoCore.cu:15-75)
@@ -331,7 +426,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     a.histo[b] = d;
                 __syncthreads();
             });
-        if (!grid_barrier(gs, gen))
+        if (!hbar(a, gs, gen, round, 0))
             goto out;
     }
 
@@ -351,6 +446,11 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         if (a.method == HM_HISTO) {
             // count_kernel (HistoCore.cu:77-128): scan bins from the old core
             // down; collapse on change
+            // A changed vertex is queued as ceil(deg/KC_HCH) wavefront-sized
+            // virtual nodes so the update phase splits hub adjacencies evenly.
+            // Changed vertices of one grab are staged in LDS and their virtual
+            // nodes reserved with a single atomic per grab.
+            uint32_t n_st = 0, st_nch = 0;
             auto one = [&](uint32_t v, uint32_t d) {
                 uint32_t old = a.core[v];
                 if (round == 1 && old <= 1)
@@ -358,15 +458,47 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 uint64_t b = hbase(a.rowptr, v);
                 uint32_t sum;
                 uint32_t k = wave_hsearch(a.histo + b + 0, old, sum);
-                bool ch = k != old;
-                if (ch && lane == 0) {
+                if (k == old)
+                    return;
+                // only neighbours with degree > k can hold a value above k
+                uint64_t r = a.nd_ds ? wave_count_gt(a.nd_ds + (b - v), d, k) : d;
+                uint32_t nch = (uint32_t)((r + KC_HCH - 1) / KC_HCH);
+                if (lane == 0) {
                     a.histo[b] = old;
                     a.histo[b + k] = sum;
                     a.core[v] = k;
+                    L.stv[wv][n_st] = v;
+                    L.stn[wv][n_st] = nch;
                 }
-                hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                n_st++;
+                st_nch += nch;
             };
-            for_items<BLOCK, R>(L, a, phase, n_in, in_list, one,
+            auto flush = [&]() {
+                if (n_st == 0)
+                    return;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                uint32_t base = 0;
+                if (lane == 0) {
+                    atomicAdd((unsigned long long *)&gs->slot[phase & 1].sum0, (unsigned long long)n_st);
+                    base = atomicAdd(&gs->slot[phase & 1].qtail, st_nch);
+                }
+                base = __shfl(base, 0, KC_WAVE);
+                if (base + st_nch > a.chunk_cap) {
+                    if (lane == 0)
+                        sync_fail(gs, 2);
+                } else {
+                    for (uint32_t i = 0; i < n_st; i++) {
+                        uint32_t v = L.stv[wv][i], nch = L.stn[wv][i];
+                        for (uint32_t c = lane; c < nch; c += KC_WAVE)
+                            a.chunks[base + c] = (uint64_t)v | ((uint64_t)c << 32);
+                        base += nch;
+                    }
+                }
+                n_st = 0;
+                st_nch = 0;
+            };
+            for_items_g<BLOCK, R>(L, a, phase, n_in, in_list, one,
                                 [&](uint32_t v) {
                                     uint32_t old = a.core[v];
                                     if (round == 1 && old <= 1)
@@ -374,16 +506,37 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                     uint64_t b = hbase(a.rowptr, v);
                                     uint32_t sum;
                                     uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
-                                    bool ch = k != old;
-                                    if (ch && threadIdx.x == 0) {
-                                        a.histo[b] = old;
-                                        a.histo[b + k] = sum;
-                                        a.core[v] = k;
+                                    if (k != old) {
+                                        uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+                                        if (wv == 0) {
+                                            uint64_t r = a.nd_ds
+                                                             ? wave_count_gt(a.nd_ds + (b - v), d, k)
+                                                             : d;
+                                            if (lane == 0)
+                                                L.bq = (uint32_t)((r + KC_HCH - 1) / KC_HCH);
+                                        }
+                                        __syncthreads();
+                                        uint32_t nch = L.bq;
+                                        if (threadIdx.x == 0) {
+                                            a.histo[b] = old;
+                                            a.histo[b + k] = sum;
+                                            a.core[v] = k;
+                                            atomicAdd((unsigned long long *)&gs->slot[phase & 1].sum0,
+                                                      1ull);
+                                            L.bk = atomicAdd(&gs->slot[phase & 1].qtail, nch);
+                                        }
+                                        __syncthreads();
+                                        uint32_t base = L.bk;
+                                        if (base + nch <= a.chunk_cap) {
+                                            for (uint32_t c = threadIdx.x; c < nch; c += BLOCK)
+                                                a.chunks[base + c] = (uint64_t)v | ((uint64_t)c << 32);
+                                        } else if (threadIdx.x == 0) {
+                                            sync_fail(gs, 2);
+                                        }
                                     }
-                                    if (wv == 0)
-                                        hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
                                     __syncthreads();
-                                });
+                                },
+                                flush);
         } else {
             // NbrCore count_kernel (NbrCore.cu:78-113) / CntCore count_kernel
             // (CntCore.cu:82-130): recompute from scratch; the new value is
@@ -394,17 +547,21 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     return round == 1 ? a.core[v] > 0 : true; // list holds cnt<core
                 return true;
             };
-            for_items<BLOCK, R>(
+            uint32_t n_st = 0;
+            for_items_g<BLOCK, R>(
                 L, a, phase, n_in, in_list,
                 [&](uint32_t v, uint32_t d) {
                     if (!want(v))
                         return;
                     uint32_t old = a.core[v];
                     uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old);
-                    bool ch = k != old;
-                    if (ch && lane == 0)
+                    if (k == old)
+                        return;
+                    if (lane == 0) {
                         a.newc[v] = k;
-                    hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                        L.stv[wv][n_st] = v;
+                    }
+                    n_st++;
                 },
                 [&](uint32_t v) {
                     if (!want(v))
@@ -420,11 +577,33 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     if (wv == 0)
                         hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
                     __syncthreads();
+                },
+                [&]() { // one list reservation per grab for the staged vertices
+                    if (n_st == 0)
+                        return;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t base = 0;
+                    if (lane == 0)
+                        base = atomicAdd(&gs->slot[phase & 1].qtail, n_st);
+                    base = __shfl(base, 0, KC_WAVE);
+                    if (lane < n_st) {
+                        if (base + lane < cap_list)
+                            c_list[base + lane] = L.stv[wv][lane];
+                        else
+                            sync_fail(gs, 2);
+                    }
+                    n_st = 0;
                 });
         }
-        if (!grid_barrier(gs, gen))
+        if (!hbar(a, gs, gen, round, 1))
             goto out;
-        uint32_t n_changed = ld_relaxed(&gs->slot[phase & 1].qtail);
+        // HistoCore counts changed vertices in sum0 and queues their virtual
+        // nodes on the chunk list; the others list changed vertices
+        uint32_t n_changed = a.method == HM_HISTO
+                                 ? (uint32_t)ld_relaxed64(&gs->slot[phase & 1].sum0)
+                                 : ld_relaxed(&gs->slot[phase & 1].qtail);
+        uint32_t n_chunks = ld_relaxed(&gs->slot[phase & 1].qtail);
         changed_total += n_changed;
         if (n_changed == 0)
             break;
@@ -432,23 +611,34 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         // ------------------------------------------------ update phase
         phase = gen;
         if (a.method == HM_HISTO) {
-            // update_histo_kernel (HistoCore.cu:130-202)
+            // update_histo_kernel (HistoCore.cu:130-202) over the virtual
+            // nodes of the changed vertices: each wave takes KC_HGRAB nodes of
+            // KC_HCH edges from the shared counter
             uint32_t *f_list = a.list[0];
-            for_items<BLOCK, R>(
-                L, a, phase, n_changed, c_list,
-                [&](uint32_t v, uint32_t d) {
-                    // lanes over the neighbours; the trip count is uniform
-                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-                    uint64_t b = hbase(a.rowptr, v);
-                    uint32_t nc = a.core[v], oc = a.histo[b];
+            for (;;) {
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd(&gs->slot[phase & 1].qhead, KC_HGRAB);
+                base = __shfl(base, 0, KC_WAVE);
+                if (base >= n_chunks)
+                    break;
+                uint32_t end = base + KC_HGRAB < n_chunks ? base + KC_HGRAB : n_chunks;
+                for (uint32_t it = base; it < end; it++) {
+                    uint64_t item = a.chunks[it];
+                    uint32_t v = (uint32_t)item;
+                    uint64_t rs = a.rowptr[v], re = a.rowptr[v + 1];
+                    uint64_t s0 = rs + (uint64_t)(uint32_t)(item >> 32) * KC_HCH;
+                    uint64_t e = s0 + KC_HCH < re ? s0 + KC_HCH : re;
+                    uint32_t nc = a.core[v], oc = a.histo[rs + v];
                     if (lane == 0)
-                        upd_edges += e - s;
-                    for (uint64_t i0 = s; i0 < e; i0 += KC_WAVE) {
+                        upd_edges += e - s0;
+                    const uint32_t *adj = a.col_ds ? a.col_ds : a.col;
+                    for (uint64_t i0 = s0; i0 < e; i0 += KC_WAVE) {
                         uint64_t i = i0 + lane;
                         bool flag = false;
                         uint32_t u = 0;
                         if (i < e) {
-                            u = a.col[i];
+                            u = adj[i];
                             uint32_t cu = a.core[u];
                             if (nc < cu) {
                                 uint64_t bu = hbase(a.rowptr, u);
@@ -460,33 +650,9 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         }
                         hpush(gs, phase, f_list, cap_list, flag, u);
                     }
-                },
-                [&](uint32_t v) {
-                    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-                    uint64_t b = hbase(a.rowptr, v);
-                    uint32_t nc = a.core[v], oc = a.histo[b];
-                    if (threadIdx.x == 0)
-                        upd_edges += e - s;
-                    for (uint64_t i0 = s; i0 < e; i0 += BLOCK) {
-                        uint64_t i = i0 + threadIdx.x;
-                        bool flag = false;
-                        uint32_t u = 0;
-                        if (i < e) {
-                            u = a.col[i];
-                            uint32_t cu = a.core[u];
-                            if (nc < cu) {
-                                uint64_t bu = hbase(a.rowptr, u);
-                                uint32_t sub = cu < oc ? cu : oc;
-                                uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
-                                atomicAdd(&a.histo[bu + nc], 1u);
-                                flag = oc >= cu && c == cu;
-                            }
-                        }
-                        hpush(gs, phase, f_list, cap_list, flag, u);
-                    }
-                    __syncthreads();
-                });
-            if (!grid_barrier(gs, gen))
+                }
+            }
+            if (!hbar(a, gs, gen, round, 2))
                 goto out;
             if (threadIdx.x == 0)
                 L.done = ld_relaxed(&gs->slot[phase & 1].qtail);
@@ -536,7 +702,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     }
                     __syncthreads();
                 });
-            if (!grid_barrier(gs, gen))
+            if (!hbar(a, gs, gen, round, 2))
                 goto out;
             uint32_t n_marked = ld_relaxed(&gs->slot[phase & 1].qtail);
             if (a.method == HM_NBR) {
@@ -598,7 +764,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     hpush(gs, phase, act_out, cap_list, keep, v);
                 }
             }
-            if (!grid_barrier(gs, gen))
+            if (!hbar(a, gs, gen, round, 3))
                 goto out;
             uint32_t n_act = ld_relaxed(&gs->slot[phase & 1].qtail);
             act_cur = act_out;
@@ -669,8 +835,14 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     a.list[1] = p.list1;
     a.list[2] = p.list2;
     a.list[3] = p.list3;
+    a.chunks = p.chunks;
+    a.chunk_cap = p.chunk_cap;
+    a.col_ds = p.col_ds;
+    a.nd_ds = p.nd_ds;
     a.gs = p.gs;
     a.stats = p.stats;
+    a.trace = p.trace;
+    a.trace_cap = p.trace_cap;
     hipLaunchKernelGGL((hindex_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V,
                        p.core, p.method == HM_CNT ? p.cnt : nullptr,
                        p.method != HM_HISTO ? p.mark : nullptr, p.gs, p.grid, p.timeout_ticks,

@@ -47,6 +47,10 @@ template <typename R> struct PeelArgs {
 
 constexpr uint32_t KC_MAXC = 1024; // live-chunk list capacity per workgroup
 constexpr uint32_t KC_DEAD = 0xffffffffu;
+constexpr uint32_t KC_AGG_LOG2 = 11; // decrement-aggregation slots per WG
+constexpr uint32_t KC_AGG = 1u << KC_AGG_LOG2;
+constexpr uint32_t KC_AGG_PROBES = 8;
+constexpr uint32_t KC_AGG_MARGIN = 32; // aggregate only targets this far above the level
 
 // LDS of one persistent workgroup.
 template <int BLOCK, int LQ> struct PeelLds {
@@ -55,6 +59,8 @@ template <int BLOCK, int LQ> struct PeelLds {
     uint32_t pend[2];     // edges pending in each local buffer
     uint64_t start[BLOCK];
     uint64_t pre[BLOCK + 1];
+    uint32_t agg_key[KC_AGG]; // decrement aggregation (see agg_add)
+    uint32_t agg_cnt[KC_AGG];
     uint32_t chunks[KC_MAXC]; // this WG's chunks that may still hold live vertices
     uint32_t nlive;           // entries of chunks[]
     uint32_t ncw;             // chunks dealt to this WG
@@ -183,6 +189,66 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
 // latency of one step is paid once for WALK_U edges.
 constexpr int WALK_U = 4;
 
+// Per-workgroup aggregation of decrements aimed at vertices whose degree is
+// far above the level (in R-MAT these are the hubs every processed vertex
+// touches; one atomic per decrement would serialise on their addresses).
+// Open addressing over KC_AGG slots, KC_AGG_PROBES probes, then the caller
+// falls back to a direct atomic.
+__device__ __forceinline__ uint32_t agg_hash(uint32_t u)
+{
+    return (u * 2654435761u) >> (32 - KC_AGG_LOG2);
+}
+
+template <int BLOCK, int LQ>
+__device__ __forceinline__ bool agg_add(PeelLds<BLOCK, LQ> &L, uint32_t u)
+{
+    uint32_t h = agg_hash(u);
+#pragma unroll
+    for (uint32_t p = 0; p < KC_AGG_PROBES; p++) {
+        uint32_t s = (h + p) & (KC_AGG - 1);
+        uint32_t k = atomicCAS(&L.agg_key[s], KC_DEAD, u);
+        if (k == KC_DEAD || k == u) {
+            atomicAdd(&L.agg_cnt[s], 1u);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Apply the summed decrements with the batched form of clamp-at-level: an
+// atomicSub of c returns old; only eff = min(c, old - level) of it stands
+// (the rest is added back), and the one subtraction whose effective result
+// lands exactly on the level moves the vertex onto it and claims it.  The
+// first subtraction to saturate sees every earlier one already settled, so
+// the effective amounts sum to min(total, degree - level) in any
+// interleaving, as the one-at-a-time rule does.
+template <int BLOCK, int LQ, typename R>
+__device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                          uint32_t phase, int pushbuf)
+{
+    for (uint32_t s0 = 0; s0 < KC_AGG; s0 += BLOCK) {
+        uint32_t s = s0 + threadIdx.x;
+        uint32_t u = L.agg_key[s];
+        bool claim = false;
+        if (u != KC_DEAD) {
+            uint32_t c = L.agg_cnt[s];
+            L.agg_key[s] = KC_DEAD;
+            L.agg_cnt[s] = 0;
+            uint32_t old = atomicSub(&a.deg[u], c);
+            uint32_t eff = old > level ? (old - level < c ? old - level : c) : 0u;
+            if (c > eff)
+                atomicAdd(&a.deg[u], c - eff);
+            if (old > level && old - eff == level && a.mode == MODE_BUCKET) {
+                uint64_t bit = 1ull << (u & 63);
+                uint64_t pv = atomicOr((unsigned long long *)&a.done[u >> 6],
+                                       (unsigned long long)bit);
+                claim = (pv & bit) == 0;
+            }
+        }
+        push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim, u);
+    }
+}
+
 template <int BLOCK, int LQ, typename R>
 __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                            uint32_t phase, uint32_t m, int pushbuf)
@@ -212,10 +278,15 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         bool hit[WALK_U];
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
-            if (a.mode == MODE_GPP)
+            if (a.mode == MODE_GPP) {
                 hit[k] = val[k] && !((pre[k] >> (u[k] & 63)) & 1ull);
-            else
+            } else {
                 hit[k] = val[k] && (uint32_t)pre[k] > level;
+                // decrements to vertices well above the level are summed in
+                // LDS and applied once per walk (flush_agg)
+                if (hit[k] && (uint32_t)pre[k] > level + KC_AGG_MARGIN && agg_add(L, u[k]))
+                    hit[k] = false;
+            }
             old[k] = hit[k] ? atomicSub(&a.deg[u[k]], 1u) : 0u;
         }
         bool claim[WALK_U];
@@ -242,6 +313,10 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
             push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim[k], u[k]);
     }
     __syncthreads();
+    if (a.mode != MODE_GPP) {
+        flush_agg<BLOCK, LQ, R>(L, a, level, phase, pushbuf);
+        __syncthreads();
+    }
 }
 
 // Block-wide exclusive scan of one u32 length per thread into L.pre.
@@ -530,6 +605,10 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         L.cnt[1] = 0;
         L.pend[0] = 0;
         L.pend[1] = 0;
+    }
+    for (uint32_t s = threadIdx.x; s < KC_AGG; s += BLOCK) {
+        L.agg_key[s] = KC_DEAD;
+        L.agg_cnt[s] = 0;
     }
     for (;;) {
         if (threadIdx.x == 0) {

@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
                 ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint64 * 3)]
+                ("prep_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 2)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_
@@ -342,6 +342,7 @@ def _trace_sig():
 
 
 KC_OPT_TRACE = 1
+KC_OPT_NO_DEGSORT = 2
 
 
 def device_trace(dgraph, max_phases=1 << 16):

@@ -103,3 +103,11 @@ def test_multigraph_from_coo_reader(ctx, method, tmp_path):
     hg = pa.HostGraph.read_coo(str(p))
     assert hg.multigraph and hg.edges_read == 8
     _check(ctx, hg.rowptr, hg.col, method)
+
+
+@pytest.mark.parametrize("flags", [0, pa.kcore.KC_OPT_NO_DEGSORT])
+def test_histocore_adjacency_order_does_not_matter(ctx, flags):
+    # the degree-ordered adjacency only prunes update work; cores, rounds and
+    # the final histogram are the same with and without it
+    hg = pa.HostGraph.chunglu(1 << 14, beta=2.1, mean_deg=20.0, cap=6000.0, seed=8)
+    _check(ctx, hg.rowptr, hg.col, "histocore", hist=True, flags=flags)
