@@ -21,6 +21,9 @@
  *  - kc_peel_dyn / kc_peelone_dyn   <- `int peel_dyn(Graph&)` (peel-dyn.cc:133) and
  *    `int peelone_dyn(Graph&)` (peelone-dyn.cc:120), the function pointers
  *    main.cu:28-29 hands to repSimulation
+ *  - kc_batch_*, kc_hb_*, kc_autotune  <- no counterpart: the reference runs
+ *    one graph per process with the fixed launch constants of
+ *    peel/inc/common.h:4-14 and peel/shared/argument.h:24
  *  - kc_vgraph_decompose            <- the device half of gppeel.cu / peelone.cu /
  *    HistoCore.cu / CntCore.cu / NbrCore.cu main(), taking the
  *    degree-prefixed VirtualGraph layout those mains build
@@ -200,6 +203,117 @@ KC_API kc_status kc_vgraph_decompose(uint32_t num_nodes, uint64_t num_edges2,
                                      const uint32_t *node_pointer, const uint32_t *edge_list,
                                      kc_method method, const kc_opts *opts, uint32_t *core_out,
                                      kc_stats *stats);
+
+/* ------------------------------------------------------------- batches */
+/* Many independent graphs in one concatenated int32 CSR.  Graph g owns
+ * vertices [vtx_off[g], vtx_off[g+1]) and adjacency entries
+ * [edge_off[g], edge_off[g+1]); its V_g+1 local row offsets (u32) start at
+ * rowptr[vtx_off[g] + g] and its col entries are local ids in [0, V_g).
+ * Cores come back concatenated the same way (vtx_off[G] entries). */
+typedef struct kc_host_batch kc_host_batch;
+typedef struct kc_batch kc_batch;
+
+typedef struct kc_batch_info {
+    uint32_t G;
+    uint32_t reserved0;
+    uint64_t vertices; /* vtx_off[G] */
+    uint64_t entries;  /* edge_off[G] */
+    const uint64_t *vtx_off;
+    const uint64_t *edge_off;
+    const uint32_t *rowptr;
+    const uint32_t *col;
+    const uint32_t *scale; /* generator draws per graph, or NULL */
+    const uint64_t *m_gen; /* generated edges per graph, or NULL */
+} kc_batch_info;
+
+/* Launch options of the batched engine; zero fields select the autotuner's
+ * pick when kc_batch_autotune has run, else defaults. */
+typedef struct kc_batch_opts {
+    uint32_t block;  /* threads per workgroup: 256, 512 or 1024 */
+    uint32_t per_cu; /* workgroups launched per CU */
+    uint32_t reserved[6];
+} kc_batch_opts;
+
+typedef struct kc_batch_stats {
+    uint64_t graphs;
+    uint64_t vertices;
+    uint64_t entries;    /* adjacency entries (2x undirected edges) */
+    uint64_t levels_run; /* levels visited, summed over graphs */
+    uint32_t kmax;       /* largest core number in the batch */
+    uint32_t block;
+    uint32_t grid;
+    uint32_t launches; /* kernel launches issued for the batch */
+    double device_ms;  /* GPU time of the launch(es) */
+    double total_ms;   /* host wall time of the call */
+    uint64_t reserved[4];
+} kc_batch_stats;
+
+/* R-MAT batch: graph g uses seed + g, scale uniform over the integers
+ * [scale_min, scale_max], edge factor uniform in [ef_min, ef_max]. */
+KC_API kc_status kc_hb_gen_rmat(uint32_t G, uint32_t scale_min, uint32_t scale_max,
+                                double ef_min, double ef_max, double a, double b, double c,
+                                uint64_t seed, kc_host_batch **out);
+/* concatenate G host graphs (copied) */
+KC_API kc_status kc_hb_from_graphs(uint32_t G, const kc_host_graph *const *graphs,
+                                   kc_host_batch **out);
+KC_API kc_status kc_hb_info(const kc_host_batch *b, kc_batch_info *info);
+/* graph g of a host batch as a stand-alone host graph */
+KC_API kc_status kc_hb_graph(const kc_host_batch *b, uint32_t g, kc_host_graph **out);
+KC_API void kc_hb_free(kc_host_batch *b);
+
+/* Copy a batch to the device (inputs caller-owned, not retained).  Every
+ * graph's rows and col entries are validated. */
+KC_API kc_status kc_batch_upload(kc_ctx *ctx, uint32_t G, const uint64_t *vtx_off,
+                                 const uint64_t *edge_off, const uint32_t *rowptr,
+                                 const uint32_t *col, kc_batch **out);
+KC_API kc_status kc_batch_upload_hb(kc_ctx *ctx, const kc_host_batch *hb, kc_batch **out);
+/* Bucket-peel core numbers of every graph in ONE persistent launch (one
+ * workgroup per graph, largest graphs first from a device work counter).
+ * core_out: vtx_off[G] entries, kmax_out: G entries; either may be NULL. */
+KC_API kc_status kc_batch_run(kc_batch *b, const kc_batch_opts *opts, uint32_t *core_out,
+                              uint32_t *kmax_out, kc_batch_stats *stats);
+/* The same batch as per-graph plain launches of kc_decompose's engines on
+ * the device-resident slices, one after another (the comparison point). */
+KC_API kc_status kc_batch_run_plain(kc_batch *b, kc_method method, const kc_opts *opts,
+                                    uint32_t *core_out, uint32_t *kmax_out,
+                                    kc_batch_stats *stats);
+/* Time every (block, per_cu) candidate on the batch itself, keep the fastest
+ * for later kc_batch_run calls and return it. */
+KC_API kc_status kc_batch_autotune(kc_batch *b, uint32_t reps, kc_batch_opts *best,
+                                   double *best_ms);
+/* upload + run + free in one call */
+KC_API kc_status kc_batch_decompose(kc_ctx *ctx, uint32_t G, const uint64_t *vtx_off,
+                                    const uint64_t *edge_off, const uint32_t *rowptr,
+                                    const uint32_t *col, const kc_batch_opts *opts,
+                                    uint32_t *core_out, kc_batch_stats *stats);
+KC_API uint64_t kc_batch_device_bytes(const kc_batch *b);
+KC_API void kc_batch_free(kc_batch *b);
+
+/* ------------------------------------------------- device-built graphs */
+/* The graphs of kc_hg_gen_rmat / kc_hg_gen_chunglu / kc_hb_gen_rmat (same
+ * seeds, same edges), generated, symmetrized, sorted and deduplicated in HBM
+ * instead of on the host.  kc_*_download copies one back for inspection. */
+KC_API kc_status kc_graph_gen_rmat(kc_ctx *ctx, uint32_t scale, uint64_t m_gen, double a,
+                                   double b, double c, uint64_t seed, uint32_t rowptr_bytes,
+                                   kc_graph **out);
+KC_API kc_status kc_graph_gen_chunglu(kc_ctx *ctx, uint32_t n, double beta, double mean_deg,
+                                      double cap, uint64_t seed, uint32_t rowptr_bytes,
+                                      kc_graph **out);
+KC_API kc_status kc_graph_download(const kc_graph *g, kc_host_graph **out);
+KC_API kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min,
+                                   uint32_t scale_max, double ef_min, double ef_max, double a,
+                                   double b, double c, uint64_t seed, kc_batch **out);
+KC_API kc_status kc_batch_download(const kc_batch *b, kc_host_batch **out);
+KC_API kc_status kc_batch_dims(const kc_batch *b, uint32_t *G, uint64_t *vertices,
+                               uint64_t *entries);
+
+/* ------------------------------------------------------------ autotuner */
+/* Time the launch candidates of `method` on graph g (block size, virtual
+ * node split width, local frontier capacity for the peels; block size for
+ * the h-index methods), `reps` runs each, and keep the fastest as the
+ * graph's default for that method; *best receives it. */
+KC_API kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *best,
+                             double *best_ms);
 
 #ifdef __cplusplus
 }

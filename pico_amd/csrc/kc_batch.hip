@@ -1,0 +1,279 @@
+// kc_batch.hip -- batched k-core: thousands of independent graphs of uneven
+// size decomposed in one launch.
+//
+// Layout (concatenated int32 CSR with per-graph offsets):
+//   vtx_off[G+1], edge_off[G+1]  u64 prefix offsets
+//   rowptr   u32, graph g's V_g+1 local offsets start at vtx_off[g] + g
+//   col      u32 local neighbour ids, graph g's at edge_off[g]
+//   core     u32 per vertex (output), at vtx_off[g]
+//   queue    u32 per vertex: the graph's peel order (each vertex enters once)
+//   live     u32 per vertex: still-unpeeled vertices, compacted per level
+//   order[G] graphs by decreasing size; a device counter hands them out
+//
+// One workgroup owns one graph at a time and runs the bucket peel of
+// peel-dyn.cc:2-131 on it with workgroup barriers only: per level, scan the
+// live list for deg == level (the scan and the cascade never overlap, so no
+// claim bits are needed), then drain the cascade in edge-balanced rounds with
+// clamp-at-level decrements.  Empty levels are skipped via the minimum live
+// degree.  Degrees live in LDS for graphs up to LDS_V vertices, otherwise in
+// the graph's slice of `core` with workgroup-scope atomics (no other
+// workgroup touches it).  No grid barrier, so the launch need not be
+// co-resident and graphs of any size mix freely.
+#include "kc_device.hpp"
+#include "kc_engines.hpp"
+
+namespace kc {
+
+template <int BLOCK, int LDS_V> struct BLds {
+    uint32_t deg[LDS_V];
+    uint64_t start[BLOCK];
+    uint64_t pre[BLOCK + 1];
+    uint32_t g, head, tail, nlive, nkeep, minv, levels;
+};
+
+__device__ __forceinline__ uint32_t wg_sub(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t wg_add(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_min_u32(uint32_t x, uint32_t *slot)
+{
+    x = wave_min_u32(x);
+    if (lane_id() == 0)
+        atomicMin(slot, x);
+    __syncthreads();
+    return *slot;
+}
+
+// Exclusive prefix of one length per thread into pre[0..m], pre[m] = total.
+template <int BLOCK, int LDS_V>
+__device__ __forceinline__ void b_stage_prefix(BLds<BLOCK, LDS_V> &L, uint32_t m, uint32_t len)
+{
+    constexpr int NW = BLOCK / KC_WAVE;
+    __shared__ uint64_t s_w[NW];
+    uint64_t incl = len;
+    uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < KC_WAVE; o <<= 1) {
+        uint64_t y = __shfl_up(incl, o, KC_WAVE);
+        if (l >= (uint32_t)o)
+            incl += y;
+    }
+    uint32_t w = threadIdx.x / KC_WAVE;
+    if (l == KC_WAVE - 1)
+        s_w[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int i = 0; i < NW; i++) {
+            uint64_t t = s_w[i];
+            s_w[i] = run;
+            run += t;
+        }
+        L.pre[m] = run;
+    }
+    __syncthreads();
+    if (threadIdx.x < m)
+        L.pre[threadIdx.x] = incl - len + s_w[w];
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t b_find(const uint64_t *pre, uint32_t n, uint64_t e)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// wave-aggregated append of `want` lanes' vertices to the graph's queue
+template <int BLOCK, int LDS_V>
+__device__ __forceinline__ void b_push(BLds<BLOCK, LDS_V> &L, uint32_t *q, bool want, uint32_t v)
+{
+    uint64_t m = ballot(want);
+    if (!m)
+        return;
+    uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(&L.tail, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, KC_WAVE);
+    if (want)
+        q[base + lane_rank(m)] = v;
+}
+
+template <int BLOCK, int LDS_V>
+__global__ __launch_bounds__(BLOCK) void batch_kernel(BatchArgs a)
+{
+    __shared__ BLds<BLOCK, LDS_V> L;
+    const uint32_t lane = lane_id();
+    for (;;) {
+        if (threadIdx.x == 0)
+            L.g = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        uint32_t gi = L.g;
+        if (gi >= a.G)
+            break;
+        const uint32_t g = a.order[gi];
+        const uint64_t voff = a.vtx_off[g];
+        const uint32_t V = (uint32_t)(a.vtx_off[g + 1] - voff);
+        const uint64_t eoff = a.edge_off[g];
+        const uint32_t *rp = a.rowptr + voff + g;
+        const uint32_t *col = a.col + eoff;
+        uint32_t *q = a.queue + voff;
+        uint32_t *live = a.live + voff;
+        const bool in_lds = V <= (uint32_t)LDS_V;
+        uint32_t *deg = in_lds ? L.deg : a.core + voff;
+        for (uint32_t v = threadIdx.x; v < V; v += BLOCK) {
+            deg[v] = rp[v + 1] - rp[v];
+            live[v] = v;
+        }
+        if (threadIdx.x == 0) {
+            L.head = 0;
+            L.tail = 0;
+            L.nlive = V;
+            L.levels = 0;
+        }
+        __syncthreads();
+        uint32_t level = 0, kmax = 0;
+        while (L.tail < V) {
+            // ---- scan the live list: claim deg == level, compact the rest
+            if (threadIdx.x == 0) {
+                L.nkeep = 0;
+                L.minv = 0xffffffffu;
+            }
+            __syncthreads();
+            const uint32_t nl = L.nlive;
+            uint32_t lmin = 0xffffffffu;
+            for (uint32_t b0 = 0; b0 < nl; b0 += BLOCK) {
+                uint32_t i = b0 + threadIdx.x;
+                uint32_t v = i < nl ? live[i] : 0u;
+                // exact values: the scan never overlaps the cascade, and HBM
+                // degrees are read past the L1 (the cascade's atomics run in L2)
+                uint32_t d = i < nl ? (in_lds ? deg[v] : ld_relaxed(&deg[v])) : 0xffffffffu;
+                bool hit = i < nl && d == level;
+                bool keep = i < nl && d > level;
+                if (keep)
+                    lmin = d < lmin ? d : lmin;
+                __syncthreads(); // every read of this block done before writes
+                b_push<BLOCK, LDS_V>(L, q, hit, v);
+                uint64_t km = ballot(keep);
+                if (km) {
+                    uint32_t leader = __ffsll((unsigned long long)km) - 1;
+                    uint32_t base = 0;
+                    if (lane == leader)
+                        base = atomicAdd(&L.nkeep, (uint32_t)__popcll(km));
+                    base = __shfl(base, leader, KC_WAVE);
+                    if (keep)
+                        live[base + lane_rank(km)] = v; // base <= b0: already read
+                }
+                __syncthreads();
+            }
+            uint32_t mn = block_min_u32<BLOCK>(lmin, &L.minv);
+            if (threadIdx.x == 0)
+                L.nlive = L.nkeep;
+            __syncthreads();
+            if (L.tail == L.head) {
+                // empty level: jump to the smallest live degree
+                level = mn;
+                continue;
+            }
+            // ---- cascade: drain the queue in edge-balanced rounds
+            while (L.head < L.tail) {
+                uint32_t h = L.head, t = L.tail;
+                uint32_t m = t - h < (uint32_t)BLOCK ? t - h : (uint32_t)BLOCK;
+                uint32_t len = 0;
+                if (threadIdx.x < m) {
+                    uint32_t v = q[h + threadIdx.x];
+                    uint32_t s = rp[v];
+                    len = rp[v + 1] - s;
+                    L.start[threadIdx.x] = s;
+                }
+                b_stage_prefix<BLOCK, LDS_V>(L, m, len);
+                if (threadIdx.x == 0)
+                    L.head = h + m;
+                const uint64_t total = L.pre[m];
+                for (uint64_t e0 = 0; e0 < total; e0 += BLOCK) {
+                    uint64_t e = e0 + threadIdx.x;
+                    bool claim = false;
+                    uint32_t u = 0;
+                    if (e < total) {
+                        uint32_t i = b_find(L.pre, m, e);
+                        u = col[L.start[i] + (e - L.pre[i])];
+                        if (deg[u] > level) {
+                            uint32_t old = in_lds ? atomicSub(&deg[u], 1u) : wg_sub(&deg[u], 1u);
+                            if (old == level + 1)
+                                claim = true;
+                            else if (old <= level) {
+                                if (in_lds)
+                                    atomicAdd(&deg[u], 1u);
+                                else
+                                    wg_add(&deg[u], 1u);
+                            }
+                        }
+                    }
+                    b_push<BLOCK, LDS_V>(L, q, claim, u);
+                }
+                __syncthreads();
+            }
+            kmax = level;
+            if (threadIdx.x == 0)
+                L.levels++;
+            level++;
+            __syncthreads();
+        }
+        if (in_lds)
+            for (uint32_t v = threadIdx.x; v < V; v += BLOCK)
+                a.core[voff + v] = L.deg[v];
+        if (threadIdx.x == 0) {
+            a.kmax[g] = V ? kmax : 0;
+            a.levels[g] = L.levels;
+        }
+        __syncthreads();
+    }
+}
+
+template <int BLOCK, int LDS_V> static hipError_t launch_batch_t(const BatchArgs &a, uint32_t grid,
+                                                                  hipStream_t st)
+{
+    hipLaunchKernelGGL((batch_kernel<BLOCK, LDS_V>), dim3(grid), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+int batch_occupancy(uint32_t block)
+{
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (block) {
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<256, 8192>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<512, 16384>, 512, 0); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<1024, 16384>, 1024, 0); break;
+    default: break;
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+uint32_t batch_lds_vertices(uint32_t block) { return block == 256 ? 8192u : 16384u; }
+
+hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st)
+{
+    switch (block) {
+    case 256: return launch_batch_t<256, 8192>(a, grid, st);
+    case 512: return launch_batch_t<512, 16384>(a, grid, st);
+    case 1024: return launch_batch_t<1024, 16384>(a, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace kc

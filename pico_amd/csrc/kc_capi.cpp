@@ -1,10 +1,6 @@
 // kc_capi.cpp -- implementation of the C ABI in include/kcore.h: host graph
 // objects, device contexts and graphs, and dispatch to the gfx950 engines.
-#include "../../include/kcore.h"
-#include "../host/kc_host.hpp"
-#include "kc_engines.hpp"
-
-#include <hip/hip_runtime.h>
+#include "kc_capi_impl.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -16,7 +12,7 @@
 #include <string>
 #include <vector>
 
-namespace {
+namespace kci {
 
 thread_local std::string g_err;
 
@@ -37,13 +33,6 @@ kc_status hip_fail(hipError_t e, const char *what)
                 hipGetErrorString(e));
 }
 
-#define KC_HIP(call, what)                                                                         \
-    do {                                                                                           \
-        hipError_t _e = (call);                                                                    \
-        if (_e != hipSuccess)                                                                      \
-            return hip_fail(_e, what);                                                             \
-    } while (0)
-
 double now_ms()
 {
     return std::chrono::duration<double, std::milli>(
@@ -51,41 +40,7 @@ double now_ms()
         .count();
 }
 
-} // namespace
-
-static constexpr uint32_t KC_TRACE_CAP = 1u << 16; // phases kept by the trace
-
-struct kc_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t num_cus = 0;
-    uint64_t hbm = 0;
-    uint64_t clock_khz = 100000; // wall_clock64 rate
-};
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
-struct kc_graph {
-    kc_ctx *ctx = nullptr;
-    uint32_t V = 0;
-    uint64_t nnz = 0;
-    bool wide = false;
-    void *rowptr = nullptr;
-    uint32_t *col = nullptr;
-    uint64_t bytes = 0;
-    // method scratch, allocated on first use and kept for reuse
-    DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
-    DevBuf col_ds, nd_ds; // degree-ordered adjacency (built on first HistoCore run)
-    bool degsorted = false;
-    uint32_t trace_phases = 0;
-    const uint32_t *result = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
-
-static kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what)
+kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what)
 {
     if (b.n >= bytes && b.p)
         return KC_OK;
@@ -103,6 +58,10 @@ static kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what)
     g->bytes += n;
     return KC_OK;
 }
+
+} // namespace kci
+
+using namespace kci;
 
 extern "C" {
 
@@ -427,8 +386,10 @@ void kc_graph_free(kc_graph *g)
         return;
     if (g->ctx)
         hipSetDevice(g->ctx->device);
-    hipFree(g->rowptr);
-    hipFree(g->col);
+    if (g->owns_csr) {
+        hipFree(g->rowptr);
+        hipFree(g->col);
+    }
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
                       &g->trace, &g->col_ds, &g->nd_ds})
@@ -714,6 +675,24 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     return KC_OK;
 }
 
+kc_status kci::run_method(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
+{
+    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    switch (method) {
+    case KC_GPP:
+    case KC_PEELONE:
+    case KC_PEEL_DYN:
+    case KC_PEELONE_DYN:
+        return run_peel(g, method, o, st);
+    case KC_HISTO:
+    case KC_CNT:
+    case KC_NBR:
+        return run_hindex(g, method, o, st);
+    default:
+        return fail(KC_EINVAL, "method %d not available", (int)method);
+    }
+}
+
 extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *opts,
                                   uint32_t *core_out, kc_stats *stats)
 {
@@ -723,24 +702,9 @@ extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *
     kc_opts o{};
     if (opts)
         o = *opts;
+    o = with_tuned(g, method, o);
     kc_stats st{};
-    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
-    kc_status s;
-    switch (method) {
-    case KC_GPP:
-    case KC_PEELONE:
-    case KC_PEEL_DYN:
-    case KC_PEELONE_DYN:
-        s = run_peel(g, method, o, st);
-        break;
-    case KC_HISTO:
-    case KC_CNT:
-    case KC_NBR:
-        s = run_hindex(g, method, o, st);
-        break;
-    default:
-        return fail(KC_EINVAL, "method %d not available", (int)method);
-    }
+    kc_status s = run_method(g, method, o, st);
     if (s)
         return s;
     if (core_out && g->V)
@@ -756,7 +720,7 @@ extern "C" kc_status kc_decompose(kc_graph *g, kc_method method, const kc_opts *
 static std::mutex g_ctx_mu;
 static kc_ctx *g_ctx0 = nullptr;
 
-static kc_ctx *default_ctx()
+kc_ctx *kci::default_ctx()
 {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     if (!g_ctx0 && kc_ctx_create(0, &g_ctx0) != KC_OK)

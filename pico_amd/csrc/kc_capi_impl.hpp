@@ -1,0 +1,84 @@
+// kc_capi_impl.hpp -- internals shared by the C-ABI translation units:
+// contexts, device graphs with their reusable method scratch, error
+// reporting and the per-method runners.
+#pragma once
+#include "../../include/kcore.h"
+#include "../host/kc_host.hpp"
+#include "kc_engines.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace kci {
+// record a message for kc_last_error() and return s
+kc_status fail(kc_status s, const char *fmt, ...);
+kc_status hip_fail(hipError_t e, const char *what);
+double now_ms();
+} // namespace kci
+
+#define KC_HIP(call, what)                                                                         \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return kci::hip_fail(_e, what);                                                        \
+    } while (0)
+
+static constexpr uint32_t KC_TRACE_CAP = 1u << 16; // phases kept by the trace
+
+struct kc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t num_cus = 0;
+    uint64_t hbm = 0;
+    uint64_t clock_khz = 100000; // wall_clock64 rate
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct kc_graph {
+    kc_ctx *ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t nnz = 0;
+    bool wide = false;
+    bool owns_csr = true; // false for a view into another object's arrays
+    void *rowptr = nullptr;
+    uint32_t *col = nullptr;
+    uint64_t bytes = 0;
+    // method scratch, allocated on first use and kept for reuse
+    DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
+    DevBuf col_ds, nd_ds; // degree-ordered adjacency (built on first HistoCore run)
+    bool degsorted = false;
+    uint32_t trace_phases = 0;
+    const uint32_t *result = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    kc_opts tuned[7] = {}; // autotuner's pick per method (block == 0: none)
+};
+
+// fill the zero fields of o from the graph's tuned options for `method`
+static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
+{
+    if ((unsigned)method >= 7 || !g->tuned[method].block)
+        return o;
+    const kc_opts &t = g->tuned[method];
+    if (!o.block)
+        o.block = t.block;
+    if (!o.hub_split)
+        o.hub_split = t.hub_split;
+    if (!o.local_cap)
+        o.local_cap = t.local_cap;
+    return o;
+}
+
+namespace kci {
+// grow b to at least `bytes` (contents are not kept)
+kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what);
+// run one method on the device; the result stays at g->result
+kc_status run_method(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st);
+// the process-wide context on device 0 used by the reference-shaped entries
+kc_ctx *default_ctx();
+} // namespace kci

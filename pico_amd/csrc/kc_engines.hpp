@@ -2,6 +2,9 @@
 // Plain structs of device pointers and sizes; no torch or STL types.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
 #include <stdint.h>
 
 namespace kc {
@@ -83,6 +86,49 @@ hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st);
 int hindex_occupancy(uint32_t block, bool wide);
 
 size_t grid_sync_bytes();
+
+// Batched decomposition (kc_batch.hip): one workgroup per graph at a time.
+struct BatchArgs {
+    uint32_t G;
+    const uint64_t *vtx_off;  // G+1
+    const uint64_t *edge_off; // G+1
+    const uint32_t *rowptr;   // per graph V_g+1 local offsets at vtx_off[g] + g
+    const uint32_t *col;      // local ids at edge_off[g]
+    uint32_t *core;           // V_total
+    uint32_t *queue;          // V_total
+    uint32_t *live;           // V_total
+    const uint32_t *order;    // G, largest first
+    uint32_t *counter;        // work counter, zeroed before launch
+    uint32_t *kmax;           // G
+    uint32_t *levels;         // G: levels visited
+};
+// Device-built synthetic graphs (kc_gen.hip): the host generators' edges,
+// sorted, symmetrized and deduplicated into the batch layout.
+enum GenKind { GEN_RMAT = 0, GEN_CHUNGLU = 1 };
+struct GenSpec {
+    GenKind kind = GEN_RMAT;
+    uint32_t G = 0;
+    const uint32_t *scale = nullptr;   // host, G (R-MAT)
+    const uint64_t *m_gen = nullptr;   // host, G: generated pairs
+    const uint64_t *vtx_off = nullptr; // host, G+1
+    uint64_t seed = 0;                 // graph g uses seed + g
+    double a = 0, ab = 0, abc = 0;     // R-MAT thresholds as the host computes them
+    const uint32_t *thr = nullptr;     // device, Chung-Lu alias table (G == 1)
+    const uint32_t *alias = nullptr;
+    bool keep_rowptr64 = false;        // also return the u64 global rowptr
+};
+struct BuildOut {
+    uint32_t *rowptr = nullptr;   // device, local u32 rows, vtx_off[G] + G
+    uint64_t *rowptr64 = nullptr; // device, global u64 rows, vtx_off[G] + 1 (optional)
+    uint32_t *col = nullptr;      // device, nnz
+    uint64_t nnz = 0;
+    std::vector<uint64_t> edge_off; // host, G+1
+};
+hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std::string &err);
+
+hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st);
+int batch_occupancy(uint32_t block);
+uint32_t batch_lds_vertices(uint32_t block);
 
 // Rows re-ordered by neighbour degree, largest first (kc_prep.hip); nd_out
 // receives the neighbour degrees; nd_tmp is nnz words of scratch.

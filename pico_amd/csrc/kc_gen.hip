@@ -1,0 +1,380 @@
+// kc_gen.hip -- synthetic graphs built on the device.
+//
+// The generators draw exactly the edges of the host generators
+// (pico_amd/host/kc_host.cpp: same counter-based hashes, same per-level
+// quadrant thresholds in double precision, same alias tables), so a graph
+// built here equals the host-built one entry for entry.  The CSR builder:
+//
+//   1. generate: edge i of graph g -> two 64-bit keys (src << 32 | dst) with
+//      src numbered from the first vertex of the chunk, self-loops -> ~0
+//   2. hipCUB radix sort of the keys (graphs are contiguous in src order, so
+//      rows come out grouped per graph and sorted by neighbour)
+//   3. hipCUB unique -> symmetric, deduplicated adjacency
+//   4. row starts from the sorted keys, col = low halves
+//
+// Graphs are processed in chunks of consecutive graphs of at most 2^30 keys
+// (two 8 GiB key buffers), so a 4,096-graph batch of ~7G adjacency entries
+// never needs more than a chunk's sort space beside its output.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "kc_engines.hpp"
+
+namespace kc {
+
+static constexpr uint64_t KEY_NONE = ~0ull;
+
+__device__ __forceinline__ uint64_t d_mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double d_u01(uint64_t h)
+{
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// last index j in [0, n) with pre[j] <= x (pre[0] == 0 <= x)
+__device__ __forceinline__ uint32_t upper_idx(const uint64_t *pre, uint32_t n, uint64_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+struct RmatKeys {
+    uint32_t gc;           // graphs in the chunk
+    const uint64_t *mpre;  // gc+1: prefix of generated edges
+    const uint32_t *scale; // gc
+    const uint64_t *gkey;  // gc: mix64(seed_g ^ tag)
+    const uint64_t *vrel;  // gc: first vertex relative to the chunk
+    double a, ab, abc;
+    uint64_t *keys;        // 2 * mpre[gc]
+};
+
+__global__ void rmat_keys_kernel(RmatKeys k)
+{
+    const uint64_t n = k.mpre[k.gc];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t g = upper_idx(k.mpre, k.gc, i);
+        uint64_t e = i - k.mpre[g];
+        uint32_t sc = k.scale[g];
+        uint64_t base = k.gkey[g] ^ (e * 0xD1B54A32D192ED03ull);
+        uint32_t u = 0, v = 0;
+        for (uint32_t l = 0; l < sc; l++) {
+            double r = d_u01(d_mix64(base + l * 0x9E3779B97F4A7C15ull));
+            uint32_t bu = r >= k.ab;
+            uint32_t bv = (r >= k.a && r < k.ab) || r >= k.abc;
+            u = (u << 1) | bu;
+            v = (v << 1) | bv;
+        }
+        uint64_t k0 = KEY_NONE, k1 = KEY_NONE;
+        if (u != v) {
+            uint64_t vr = k.vrel[g];
+            k0 = ((vr + u) << 32) | v;
+            k1 = ((vr + v) << 32) | u;
+        }
+        k.keys[2 * i] = k0;
+        k.keys[2 * i + 1] = k1;
+    }
+}
+
+struct AliasKeys {
+    uint64_t m;            // pairs to draw
+    uint32_t n;            // vertices
+    uint64_t key;          // mix64(seed ^ tag)
+    const uint32_t *thr;   // n: 32-bit acceptance thresholds
+    const uint32_t *alias; // n
+    uint64_t *keys;        // 2m
+};
+
+__device__ __forceinline__ uint32_t alias_draw(const AliasKeys &k, uint64_t h)
+{
+    uint32_t i = (uint32_t)(((h >> 32) * (uint64_t)k.n) >> 32);
+    return ((uint32_t)h < k.thr[i]) ? i : k.alias[i];
+}
+
+__global__ void alias_keys_kernel(AliasKeys k)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k.m; i += stride) {
+        uint64_t base = k.key ^ (i * 0xD1B54A32D192ED03ull);
+        uint32_t s = alias_draw(k, d_mix64(base));
+        uint32_t d = alias_draw(k, d_mix64(base + 0x9E3779B97F4A7C15ull));
+        uint64_t k0 = KEY_NONE, k1 = KEY_NONE;
+        if (s != d) {
+            k0 = ((uint64_t)s << 32) | d;
+            k1 = ((uint64_t)d << 32) | s;
+        }
+        k.keys[2 * i] = k0;
+        k.keys[2 * i + 1] = k1;
+    }
+}
+
+// Row starts of the chunk's vertices [0, nv] from n sorted unique keys (the
+// last may be KEY_NONE) and the col entries, written at offset `base`.
+__global__ void rows_from_keys(const uint64_t *keys, uint64_t n, uint32_t nv, uint64_t base,
+                               uint32_t *col, uint64_t *rp)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        uint64_t key = i < n ? keys[i] : KEY_NONE;
+        uint64_t s = key == KEY_NONE ? nv : (key >> 32);
+        uint64_t p = 0;
+        if (i > 0) {
+            uint64_t pk = keys[i - 1];
+            p = (pk == KEY_NONE ? (uint64_t)nv : (pk >> 32)) + 1;
+        }
+        for (uint64_t w = p; w <= s && w <= nv; w++)
+            rp[w] = base + i;
+        if (key != KEY_NONE)
+            col[base + i] = (uint32_t)key;
+    }
+}
+
+// local u32 rows: entry j of graph g (j in [0, V_g]) = rp[vo[g] + j] - rp[vo[g]]
+__global__ void local_rows(const uint64_t *rp, const uint64_t *vo, uint32_t G, uint32_t *out)
+{
+    const uint64_t total = vo[G] + G;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += stride) {
+        // graph g owns output slots [vo[g] + g, vo[g+1] + g + 1)
+        uint32_t lo = 0, hi = G;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (vo[mid] + mid <= j)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        uint64_t v = j - vo[lo] - lo;
+        out[j] = (uint32_t)(rp[vo[lo] + v] - rp[vo[lo]]);
+    }
+}
+
+__global__ void gather_edge_off(const uint64_t *rp, const uint64_t *vo, uint32_t G, uint64_t *eo)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x)
+        eo[g] = rp[vo[g]];
+}
+
+static inline uint64_t h_mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+static uint32_t bits_for(uint64_t x)
+{
+    uint32_t b = 1;
+    while (b < 64 && (x >> b))
+        b++;
+    return b;
+}
+
+namespace {
+struct Scratch {
+    std::vector<void *> ptrs;
+    ~Scratch()
+    {
+        for (void *p : ptrs)
+            hipFree(p);
+    }
+    template <typename T> hipError_t get(T **p, size_t n)
+    {
+        hipError_t e = hipMalloc((void **)p, n ? n * sizeof(T) : sizeof(T));
+        if (e == hipSuccess)
+            ptrs.push_back(*p);
+        return e;
+    }
+};
+} // namespace
+
+#define GEN_TRY(x)                                                                                 \
+    do {                                                                                           \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) {                                                                    \
+            err = std::string(#x) + ": " + hipGetErrorString(_e);                                  \
+            return _e;                                                                             \
+        }                                                                                          \
+    } while (0)
+
+hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std::string &err)
+{
+    const uint32_t G = s.G;
+    const uint64_t *vo = s.vtx_off;
+    const uint64_t Vtot = vo[G];
+    constexpr uint64_t CHUNK_KEYS = 1ull << 30;
+    // chunks of consecutive graphs
+    std::vector<uint32_t> cut{0};
+    uint64_t acc = 0, total_pairs = 0;
+    for (uint32_t g = 0; g < G; g++) {
+        uint64_t k = 2 * s.m_gen[g];
+        if (k > CHUNK_KEYS) {
+            err = "a graph has more than 2^29 generated edges";
+            return hipErrorInvalidValue;
+        }
+        if (vo[g + 1] - vo[g] > 0xffffffffull) {
+            err = "a graph has 2^32 or more vertices";
+            return hipErrorInvalidValue;
+        }
+        if (acc + k > CHUNK_KEYS || (cut.size() && g > cut.back() &&
+                                     vo[g + 1] - vo[cut.back()] > 0x7fffffffull)) {
+            cut.push_back(g);
+            acc = 0;
+        }
+        acc += k;
+        total_pairs += s.m_gen[g];
+    }
+    cut.push_back(G);
+    uint64_t max_keys = 0;
+    for (size_t c = 0; c + 1 < cut.size(); c++) {
+        uint64_t k = 0;
+        for (uint32_t g = cut[c]; g < cut[c + 1]; g++)
+            k += 2 * s.m_gen[g];
+        max_keys = std::max(max_keys, k);
+    }
+    Scratch sc;
+    uint64_t *k0 = nullptr, *k1 = nullptr, *rp = nullptr, *d_vo = nullptr, *d_meta = nullptr;
+    uint32_t *col_big = nullptr, *d_scale = nullptr;
+    int *d_nsel = nullptr;
+    GEN_TRY(sc.get(&k0, max_keys));
+    GEN_TRY(sc.get(&k1, max_keys));
+    GEN_TRY(sc.get(&rp, Vtot + 1));
+    GEN_TRY(sc.get(&d_vo, G + 1ull));
+    GEN_TRY(sc.get(&d_meta, 3ull * G + 1));
+    GEN_TRY(sc.get(&d_scale, G ? G : 1));
+    GEN_TRY(sc.get(&d_nsel, 1));
+    GEN_TRY(hipMalloc((void **)&col_big, (2 * total_pairs ? 2 * total_pairs : 1) * 4));
+    struct Free {
+        uint32_t *&p;
+        ~Free()
+        {
+            if (p)
+                hipFree(p);
+        }
+    } free_big{col_big};
+    GEN_TRY(hipMemcpyAsync(d_vo, vo, (G + 1ull) * 8, hipMemcpyHostToDevice, st));
+    // sort/unique temp space sized once for the largest chunk
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    if (max_keys) {
+        hipcub::DoubleBuffer<uint64_t> db(k0, k1);
+        size_t need = 0, need2 = 0;
+        GEN_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, need, db, (int)max_keys, 0, 64, st));
+        GEN_TRY(hipcub::DeviceSelect::Unique(nullptr, need2, k0, k1, d_nsel, (int)max_keys, st));
+        tmp_bytes = std::max(need, need2);
+        GEN_TRY(sc.get((uint8_t **)&tmp, tmp_bytes));
+    }
+    // per-graph metadata for the R-MAT kernel: mpre (chunk-relative, G+1),
+    // gkey (G), vrel (G)
+    std::vector<uint64_t> meta(3ull * G + 1);
+    std::vector<uint32_t> scale_h(G);
+    uint64_t base = 0;
+    for (size_t c = 0; c + 1 < cut.size(); c++) {
+        uint32_t g0 = cut[c], g1 = cut[c + 1], gc = g1 - g0;
+        uint64_t v0 = vo[g0];
+        uint32_t nv = (uint32_t)(vo[g1] - v0);
+        uint64_t npairs = 0;
+        uint64_t *mpre = meta.data(), *gkey = meta.data() + gc + 1, *vrel = gkey + gc;
+        for (uint32_t j = 0; j < gc; j++) {
+            uint32_t g = g0 + j;
+            mpre[j] = npairs;
+            npairs += s.m_gen[g];
+            gkey[j] = h_mix64((s.seed + g) ^ 0x524d41542d67656eull);
+            vrel[j] = vo[g] - v0;
+            scale_h[j] = s.kind == GEN_RMAT ? s.scale[g] : 0;
+        }
+        mpre[gc] = npairs;
+        uint64_t nkeys = 2 * npairs;
+        if (npairs) {
+            if (s.kind == GEN_RMAT) {
+                GEN_TRY(hipMemcpyAsync(d_meta, meta.data(), (3ull * gc + 1) * 8,
+                                       hipMemcpyHostToDevice, st));
+                GEN_TRY(hipMemcpyAsync(d_scale, scale_h.data(), (size_t)gc * 4,
+                                       hipMemcpyHostToDevice, st));
+                RmatKeys k{};
+                k.gc = gc;
+                k.mpre = d_meta;
+                k.gkey = d_meta + gc + 1;
+                k.vrel = d_meta + 2 * gc + 1;
+                k.scale = d_scale;
+                k.a = s.a;
+                k.ab = s.ab;
+                k.abc = s.abc;
+                k.keys = k0;
+                hipLaunchKernelGGL(rmat_keys_kernel, dim3(8192), dim3(256), 0, st, k);
+            } else {
+                AliasKeys k{};
+                k.m = npairs;
+                k.n = nv;
+                k.key = h_mix64(s.seed ^ 0x4368756e674c7521ull);
+                k.thr = s.thr;
+                k.alias = s.alias;
+                k.keys = k0;
+                hipLaunchKernelGGL(alias_keys_kernel, dim3(8192), dim3(256), 0, st, k);
+            }
+            GEN_TRY(hipGetLastError());
+            // sort on the bits in use; KEY_NONE keeps every bit set, so it
+            // still sorts last
+            int end_bit = (int)std::min<uint32_t>(64, 32 + bits_for(nv));
+            hipcub::DoubleBuffer<uint64_t> db(k0, k1);
+            GEN_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, db, (int)nkeys, 0, end_bit,
+                                                      st));
+            uint64_t *sorted = db.Current(), *uniq = db.Alternate();
+            GEN_TRY(hipcub::DeviceSelect::Unique(tmp, tmp_bytes, sorted, uniq, d_nsel, (int)nkeys,
+                                                 st));
+            int nsel = 0;
+            GEN_TRY(hipMemcpyAsync(&nsel, d_nsel, 4, hipMemcpyDeviceToHost, st));
+            GEN_TRY(hipStreamSynchronize(st));
+            hipLaunchKernelGGL(rows_from_keys, dim3(4096), dim3(256), 0, st, uniq, (uint64_t)nsel,
+                               nv, base, col_big, rp + v0);
+        } else {
+            hipLaunchKernelGGL(rows_from_keys, dim3(1), dim3(256), 0, st, k0, (uint64_t)0, nv,
+                               base, col_big, rp + v0);
+        }
+        GEN_TRY(hipGetLastError());
+        uint64_t end = 0;
+        GEN_TRY(hipMemcpyAsync(&end, rp + v0 + nv, 8, hipMemcpyDeviceToHost, st));
+        GEN_TRY(hipStreamSynchronize(st));
+        base = end;
+    }
+    // exact-size outputs
+    out.nnz = base;
+    GEN_TRY(hipMalloc((void **)&out.col, base ? base * 4 : 4));
+    if (base)
+        GEN_TRY(hipMemcpyAsync(out.col, col_big, base * 4, hipMemcpyDeviceToDevice, st));
+    GEN_TRY(hipMalloc((void **)&out.rowptr, (Vtot + G) * 4 + 4));
+    if (Vtot + G)
+        hipLaunchKernelGGL(local_rows, dim3(4096), dim3(256), 0, st, rp, d_vo, G, out.rowptr);
+    GEN_TRY(hipGetLastError());
+    uint64_t *d_eo = d_meta; // >= G+1 words
+    hipLaunchKernelGGL(gather_edge_off, dim3((G + 256) / 256), dim3(256), 0, st, rp, d_vo, G,
+                       d_eo);
+    GEN_TRY(hipGetLastError());
+    out.edge_off.resize(G + 1ull);
+    GEN_TRY(hipMemcpyAsync(out.edge_off.data(), d_eo, (G + 1ull) * 8, hipMemcpyDeviceToHost, st));
+    if (s.keep_rowptr64) {
+        GEN_TRY(hipMalloc((void **)&out.rowptr64, (Vtot + 1) * 8));
+        GEN_TRY(hipMemcpyAsync(out.rowptr64, rp, (Vtot + 1) * 8, hipMemcpyDeviceToDevice, st));
+    }
+    GEN_TRY(hipStreamSynchronize(st));
+    return hipSuccess;
+}
+
+} // namespace kc

@@ -23,28 +23,9 @@ unsigned worker_threads()
     return n > 16 ? 16 : n; // the GPU hosts give one GPU a 16-core share
 }
 
-template <typename F> void parallel_for(uint64_t n, uint64_t chunk, F &&f)
-{
-    unsigned T = worker_threads();
-    if (n <= chunk || T == 1) {
-        for (uint64_t i = 0; i < n; i += chunk)
-            f(i, std::min(n, i + chunk));
-        return;
-    }
-    std::atomic<uint64_t> next{0};
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < T; t++)
-        th.emplace_back([&] {
-            for (;;) {
-                uint64_t b = next.fetch_add(chunk);
-                if (b >= n)
-                    break;
-                f(b, std::min(n, b + chunk));
-            }
-        });
-    for (auto &x : th)
-        x.join();
-}
+static thread_local bool tl_serial = false;
+void set_serial(bool on) { tl_serial = on; }
+bool serial() { return tl_serial; }
 
 // ---------------------------------------------------------------- CSR -----
 // Symmetrize, drop self-loops, dedup and sort every row.  Row order of the
@@ -160,8 +141,8 @@ int gen_rmat(uint32_t scale, uint64_t m_gen, double a, double b, double c, uint6
 // Chung-Lu with power-law weights w_i ~ (i+1)^(-1/(beta-1)), scaled to the
 // requested mean degree and capped; n*mean/2 endpoint pairs drawn
 // independently proportional to w (Vose alias table), then dedup/loops.
-int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t seed,
-                kc_host_graph &g)
+int chunglu_alias(uint32_t n, double beta, double mean_deg, double cap,
+                  std::vector<uint32_t> &thr, std::vector<uint32_t> &alias, uint64_t &m_gen)
 {
     if (n < 2 || beta <= 1.0)
         return -1;
@@ -184,8 +165,8 @@ int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t s
         total += w[i];
     }
     // Vose alias table
-    std::vector<uint32_t> alias(n);
-    std::vector<uint32_t> thr(n);
+    alias.assign(n, 0);
+    thr.assign(n, 0);
     std::vector<double> p(n);
     std::vector<uint32_t> small, large;
     for (uint32_t i = 0; i < n; i++) {
@@ -211,7 +192,17 @@ int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t s
         thr[i] = 0xffffffffu;
         alias[i] = i;
     }
-    uint64_t m_gen = (uint64_t)std::llround(mean_deg * (double)n / 2.0);
+    m_gen = (uint64_t)std::llround(mean_deg * (double)n / 2.0);
+    return 0;
+}
+
+int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t seed,
+                kc_host_graph &g)
+{
+    std::vector<uint32_t> thr, alias;
+    uint64_t m_gen = 0;
+    if (chunglu_alias(n, beta, mean_deg, cap, thr, alias, m_gen))
+        return -1;
     std::vector<uint32_t> s(m_gen), d(m_gen);
     uint64_t key = mix64(seed ^ 0x4368756e674c7521ull);
     auto draw = [&](uint64_t h) -> uint32_t {
@@ -226,6 +217,88 @@ int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t s
         }
     });
     return csr_from_pairs(n, m_gen, s.data(), d.data(), g);
+}
+
+// -------------------------------------------------------------- batches -----
+int batch_from_graphs(std::vector<kc_host_graph> &gs, kc_host_batch &out, bool release)
+{
+    uint32_t G = (uint32_t)gs.size();
+    out.G = G;
+    out.vtx_off.assign((size_t)G + 1, 0);
+    out.edge_off.assign((size_t)G + 1, 0);
+    for (uint32_t g = 0; g < G; g++) {
+        if (gs[g].rowptr.size() != (size_t)gs[g].V + 1 || gs[g].rowptr[gs[g].V] > 0xffffffffull)
+            return -1;
+        out.vtx_off[g + 1] = out.vtx_off[g] + gs[g].V;
+        out.edge_off[g + 1] = out.edge_off[g] + gs[g].rowptr[gs[g].V];
+    }
+    out.rowptr.resize(out.vtx_off[G] + G);
+    out.col.resize(out.edge_off[G]);
+    parallel_for(G, 1, [&](uint64_t b, uint64_t e) {
+        for (uint64_t g = b; g < e; g++) {
+            kc_host_graph &h = gs[g];
+            uint32_t *rp = out.rowptr.data() + out.vtx_off[g] + g;
+            for (uint64_t v = 0; v <= h.V; v++)
+                rp[v] = (uint32_t)h.rowptr[v];
+            if (!h.col.empty())
+                std::memcpy(out.col.data() + out.edge_off[g], h.col.data(), h.col.size() * 4);
+            if (release) {
+                std::vector<uint64_t>().swap(h.rowptr);
+                std::vector<uint32_t>().swap(h.col);
+            }
+        }
+    });
+    return 0;
+}
+
+void rmat_batch_draws(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef_min,
+                      double ef_max, uint64_t seed, std::vector<uint32_t> &sc,
+                      std::vector<uint64_t> &mg)
+{
+    sc.resize(G);
+    mg.resize(G);
+    uint32_t span = scale_max - scale_min + 1;
+    uint64_t key = mix64(seed ^ 0x62617463682d726dull);
+    for (uint32_t g = 0; g < G; g++) {
+        uint64_t h = mix64(key ^ ((uint64_t)g * 0xD1B54A32D192ED03ull));
+        sc[g] = scale_min + (uint32_t)(((h >> 32) * (uint64_t)span) >> 32);
+        double ef = ef_min + u01(mix64(h)) * (ef_max - ef_min);
+        mg[g] = (uint64_t)std::llround(ef * (double)(1ull << sc[g]));
+    }
+}
+
+int gen_rmat_batch(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef_min,
+                   double ef_max, double a, double b, double c, uint64_t seed, kc_host_batch &out)
+{
+    if (scale_min == 0 || scale_max > 31 || scale_min > scale_max || ef_min < 0 ||
+        ef_max < ef_min)
+        return -1;
+    std::vector<uint32_t> sc;
+    std::vector<uint64_t> mg;
+    rmat_batch_draws(G, scale_min, scale_max, ef_min, ef_max, seed, sc, mg);
+    // largest first so the workers finish together
+    std::vector<uint32_t> ord(G);
+    for (uint32_t g = 0; g < G; g++)
+        ord[g] = g;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return mg[x] > mg[y]; });
+    std::vector<kc_host_graph> gs(G);
+    std::atomic<int> bad{0};
+    parallel_for(G, 1, [&](uint64_t lo, uint64_t hi) {
+        set_serial(true);
+        for (uint64_t i = lo; i < hi; i++) {
+            uint32_t g = ord[i];
+            if (gen_rmat(sc[g], mg[g], a, b, c, seed + g, gs[g]))
+                bad = 1;
+        }
+        set_serial(false);
+    });
+    if (bad)
+        return -1;
+    if (batch_from_graphs(gs, out, true))
+        return -1;
+    out.scale = sc;
+    out.m_gen = mg;
+    return 0;
 }
 
 // ------------------------------------------------------------- readers -----

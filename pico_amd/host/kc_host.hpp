@@ -3,7 +3,9 @@
 #pragma once
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 struct kc_host_graph {
@@ -16,12 +18,51 @@ struct kc_host_graph {
     bool multigraph = false; // COO reader keeps duplicate edges
 };
 
+// Many independent graphs in one concatenated int32 CSR: graph g owns
+// vertices [vtx_off[g], vtx_off[g+1]) and adjacency entries
+// [edge_off[g], edge_off[g+1]); its V_g+1 local row offsets start at
+// rowptr[vtx_off[g] + g] and its col entries are local ids in [0, V_g).
+struct kc_host_batch {
+    uint32_t G = 0;
+    std::vector<uint64_t> vtx_off, edge_off; // G+1 each
+    std::vector<uint32_t> rowptr;            // vtx_off[G] + G
+    std::vector<uint32_t> col;               // edge_off[G]
+    std::vector<uint32_t> scale;             // generator draws (empty when built from arrays)
+    std::vector<uint64_t> m_gen;
+};
+
 namespace kch {
 
 unsigned worker_threads();
+// while set on this thread, parallel_for runs inline (outer loops that are
+// already parallel over independent work items)
+void set_serial(bool on);
+bool serial();
 
-// parallel-for over [0, n) in chunks
-template <typename F> void parallel_for(uint64_t n, uint64_t chunk, F &&f);
+// parallel-for over [0, n) in chunks of `chunk`, on worker_threads() threads
+template <typename F> void parallel_for(uint64_t n, uint64_t chunk, F &&f)
+{
+    unsigned T = serial() ? 1 : worker_threads();
+    if (n <= chunk || T == 1) {
+        for (uint64_t i = 0; i < n; i += chunk)
+            f(i, n < i + chunk ? n : i + chunk);
+        return;
+    }
+    std::atomic<uint64_t> next{0};
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (;;) {
+                uint64_t b = next.fetch_add(chunk);
+                if (b >= n)
+                    break;
+                f(b, n < b + chunk ? n : b + chunk);
+            }
+        });
+    for (auto &x : th)
+        x.join();
+}
+
 
 // counter-based RNG (splitmix64 finaliser)
 static inline uint64_t mix64(uint64_t x)
@@ -42,6 +83,22 @@ int gen_rmat(uint32_t scale, uint64_t m_gen, double a, double b, double c, uint6
              kc_host_graph &g);
 int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t seed,
                 kc_host_graph &g);
+// the Chung-Lu generator's alias table (32-bit thresholds) and pair count
+int chunglu_alias(uint32_t n, double beta, double mean_deg, double cap,
+                  std::vector<uint32_t> &thr, std::vector<uint32_t> &alias, uint64_t &m_gen);
+// per-graph (scale, generated pairs) of an R-MAT batch
+void rmat_batch_draws(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef_min,
+                      double ef_max, uint64_t seed, std::vector<uint32_t> &sc,
+                      std::vector<uint64_t> &mg);
+
+// 4096-style batch of R-MAT graphs: graph g uses seed + g, scale drawn
+// uniformly from the integers [scale_min, scale_max] and edge factor
+// uniformly from [ef_min, ef_max] (both from a hash of seed and g).
+int gen_rmat_batch(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef_min,
+                   double ef_max, double a, double b, double c, uint64_t seed, kc_host_batch &out);
+// concatenate independent host graphs into the batch layout; with release
+// each graph's arrays are freed once copied
+int batch_from_graphs(std::vector<kc_host_graph> &gs, kc_host_batch &out, bool release);
 
 int read_coo(const char *path, kc_host_graph &g, std::string &err);
 int read_dedup(const char *path, kc_host_graph &g, std::string &err);

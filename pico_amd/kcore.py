@@ -70,6 +70,35 @@ class RefGraph(ctypes.Structure):
                 ("kmax", ctypes.c_uint32), ("dmax", ctypes.c_uint32)]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("vertices", ctypes.c_uint64), ("entries", ctypes.c_uint64),
+                ("vtx_off", ctypes.POINTER(ctypes.c_uint64)),
+                ("edge_off", ctypes.POINTER(ctypes.c_uint64)),
+                ("rowptr", ctypes.POINTER(ctypes.c_uint32)),
+                ("col", ctypes.POINTER(ctypes.c_uint32)),
+                ("scale", ctypes.POINTER(ctypes.c_uint32)),
+                ("m_gen", ctypes.POINTER(ctypes.c_uint64))]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("per_cu", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
+                ("entries", ctypes.c_uint64), ("levels_run", ctypes.c_uint64),
+                ("kmax", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("grid", ctypes.c_uint32), ("launches", ctypes.c_uint32),
+                ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if not k.startswith("reserved")}
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -115,6 +144,40 @@ def lib():
         "kc_peelone_dyn": ([ctypes.POINTER(RefGraph), _vp], ctypes.c_int),
         "kc_vgraph_decompose": ([ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, ctypes.c_int,
                                  ctypes.POINTER(Opts), _vp, ctypes.POINTER(Stats)], S),
+        "kc_hb_gen_rmat": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_uint64, ctypes.POINTER(_vp)], S),
+        "kc_hb_from_graphs": ([ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], S),
+        "kc_hb_info": ([_vp, ctypes.POINTER(BatchInfo)], S),
+        "kc_hb_graph": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp)], S),
+        "kc_hb_free": ([_vp], None),
+        "kc_batch_upload": ([_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)], S),
+        "kc_batch_upload_hb": ([_vp, _vp, ctypes.POINTER(_vp)], S),
+        "kc_batch_run": ([_vp, ctypes.POINTER(BatchOpts), _vp, _vp,
+                          ctypes.POINTER(BatchStats)], S),
+        "kc_batch_run_plain": ([_vp, ctypes.c_int, ctypes.POINTER(Opts), _vp, _vp,
+                                ctypes.POINTER(BatchStats)], S),
+        "kc_batch_autotune": ([_vp, ctypes.c_uint32, ctypes.POINTER(BatchOpts),
+                               ctypes.POINTER(ctypes.c_double)], S),
+        "kc_batch_decompose": ([_vp, ctypes.c_uint32, _vp, _vp, _vp, _vp,
+                                ctypes.POINTER(BatchOpts), _vp, ctypes.POINTER(BatchStats)], S),
+        "kc_batch_device_bytes": ([_vp], ctypes.c_uint64),
+        "kc_batch_free": ([_vp], None),
+        "kc_autotune": ([_vp, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(Opts),
+                         ctypes.POINTER(ctypes.c_double)], S),
+        "kc_graph_gen_rmat": ([_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                               ctypes.c_uint32, ctypes.POINTER(_vp)], S),
+        "kc_graph_gen_chunglu": ([_vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                                  ctypes.POINTER(_vp)], S),
+        "kc_graph_download": ([_vp, ctypes.POINTER(_vp)], S),
+        "kc_batch_gen_rmat": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                               ctypes.POINTER(_vp)], S),
+        "kc_batch_download": ([_vp, ctypes.POINTER(_vp)], S),
+        "kc_batch_dims": ([_vp, _u32p, _u64p, _u64p], S),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -237,6 +300,80 @@ class HostGraph:
         return cls._make(lib().kc_hg_gen_chunglu, n, beta, mean_deg, cap, seed)
 
 
+class HostBatch:
+    """Many independent graphs in the concatenated layout (kc_host_batch)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = BatchInfo()
+        _check(lib().kc_hb_info(self._h, ctypes.byref(info)))
+        self.G = int(info.G)
+        self.vertices = int(info.vertices)
+        self.entries = int(info.entries)
+        self._info = info
+
+    def _arr(self, ptr, n):
+        if n == 0 or not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    @property
+    def vtx_off(self):
+        return self._arr(self._info.vtx_off, self.G + 1).copy()
+
+    @property
+    def edge_off(self):
+        return self._arr(self._info.edge_off, self.G + 1).copy()
+
+    @property
+    def scales(self):
+        a = self._arr(self._info.scale, self.G)
+        return None if a is None else a.copy()
+
+    @property
+    def m_gen(self):
+        a = self._arr(self._info.m_gen, self.G)
+        return None if a is None else a.copy()
+
+    def arrays(self):
+        """(vtx_off, edge_off, rowptr, col) views into the batch's memory."""
+        return (self._arr(self._info.vtx_off, self.G + 1),
+                self._arr(self._info.edge_off, self.G + 1),
+                self._arr(self._info.rowptr, self.vertices + self.G),
+                self._arr(self._info.col, self.entries))
+
+    def graph(self, g):
+        h = _vp()
+        _check(lib().kc_hb_graph(self._h, g, ctypes.byref(h)))
+        return HostGraph(h)
+
+    def close(self):
+        if self._h:
+            lib().kc_hb_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def rmat(cls, G, scale_min=8, scale_max=18, ef_min=4.0, ef_max=32.0, a=0.57, b=0.19,
+             c=0.19, seed=1):
+        h = _vp()
+        _check(lib().kc_hb_gen_rmat(G, scale_min, scale_max, ef_min, ef_max, a, b, c, seed,
+                                    ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_graphs(cls, graphs):
+        arr = (_vp * max(len(graphs), 1))(*[g._h for g in graphs])
+        h = _vp()
+        _check(lib().kc_hb_from_graphs(len(graphs), arr, ctypes.byref(h)))
+        return cls(h)
+
+
 # ---------------------------------------------------------------- device
 class Context:
     def __init__(self, device=0):
@@ -258,6 +395,29 @@ class Context:
         _check(lib().kc_graph_upload_hg(self._h, graph._h, rowptr_bytes, ctypes.byref(h)))
         return DeviceGraph(self, h, graph.V)
 
+    def gen_rmat(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, rowptr_bytes=4):
+        """HostGraph.rmat's graph, built in HBM."""
+        m_gen = int(round(edge_factor * (1 << scale)))
+        h = _vp()
+        _check(lib().kc_graph_gen_rmat(self._h, scale, m_gen, a, b, c, seed, rowptr_bytes,
+                                       ctypes.byref(h)))
+        return DeviceGraph(self, h, 1 << scale)
+
+    def gen_chunglu(self, n, beta=2.1, mean_deg=20.0, cap=4e6, seed=3, rowptr_bytes=4):
+        """HostGraph.chunglu's graph, built in HBM."""
+        h = _vp()
+        _check(lib().kc_graph_gen_chunglu(self._h, n, beta, mean_deg, cap, seed, rowptr_bytes,
+                                          ctypes.byref(h)))
+        return DeviceGraph(self, h, n)
+
+    def gen_rmat_batch(self, G, scale_min=8, scale_max=18, ef_min=4.0, ef_max=32.0, a=0.57,
+                       b=0.19, c=0.19, seed=1):
+        """HostBatch.rmat's batch, built in HBM."""
+        h = _vp()
+        _check(lib().kc_batch_gen_rmat(self._h, G, scale_min, scale_max, ef_min, ef_max, a, b,
+                                       c, seed, ctypes.byref(h)))
+        return DeviceBatch(self, handle=h)
+
     def upload_csr(self, rowptr, col, rowptr_bytes=4):
         rp = np.ascontiguousarray(rowptr, np.uint64)
         cl = np.ascontiguousarray(col, np.uint32)
@@ -268,10 +428,24 @@ class Context:
 
 
 class DeviceGraph:
+    def autotune(self, method="peelone_dyn", reps=1):
+        """Time the launch candidates of `method` on this graph and keep the
+        fastest as the graph's default; returns (opts dict, ms)."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        o = Opts()
+        ms = ctypes.c_double(0)
+        _check(lib().kc_autotune(self._h, m, reps, ctypes.byref(o), ctypes.byref(ms)))
+        return {"block": o.block, "hub_split": o.hub_split, "local_cap": o.local_cap}, ms.value
+
     def __init__(self, ctx, handle, V):
         self.ctx = ctx
         self._h = handle
         self.V = V
+
+    def download(self):
+        h = _vp()
+        _check(lib().kc_graph_download(self._h, ctypes.byref(h)))
+        return HostGraph(h)
 
     @property
     def device_bytes(self):
@@ -288,6 +462,80 @@ class DeviceGraph:
     def close(self):
         if self._h:
             lib().kc_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBatch:
+    """A batch resident on the device (kc_batch)."""
+
+    def __init__(self, ctx, hb=None, arrays=None, handle=None):
+        self.ctx = ctx
+        h = _vp()
+        if handle is not None:
+            h = handle
+            G = ctypes.c_uint32(0)
+            nv = ctypes.c_uint64(0)
+            _check(lib().kc_batch_dims(h, ctypes.byref(G), ctypes.byref(nv), None))
+            self.G, self.vertices = G.value, nv.value
+        elif hb is not None:
+            _check(lib().kc_batch_upload_hb(ctx._h, hb._h, ctypes.byref(h)))
+            self.G, self.vertices = hb.G, hb.vertices
+        else:
+            vo, eo, rp, cl = [np.ascontiguousarray(x, t) for x, t in
+                              zip(arrays, (np.uint64, np.uint64, np.uint32, np.uint32))]
+            _check(lib().kc_batch_upload(ctx._h, len(vo) - 1, _ptr(vo), _ptr(eo), _ptr(rp),
+                                         _ptr(cl), ctypes.byref(h)))
+            self.G, self.vertices = len(vo) - 1, int(vo[-1])
+        self._h = h
+
+    @property
+    def device_bytes(self):
+        return int(lib().kc_batch_device_bytes(self._h))
+
+    def run(self, block=0, per_cu=0, copy=True):
+        o = BatchOpts()
+        o.block, o.per_cu = block, per_cu
+        st = BatchStats()
+        core = np.zeros(max(self.vertices, 1), np.uint32) if copy else None
+        km = np.zeros(max(self.G, 1), np.uint32) if copy else None
+        _check(lib().kc_batch_run(self._h, ctypes.byref(o), _ptr(core), _ptr(km),
+                                  ctypes.byref(st)))
+        if not copy:
+            return None, None, st.as_dict()
+        return core[:self.vertices], km[:self.G], st.as_dict()
+
+    def run_plain(self, method="peelone_dyn", copy=True, **opts):
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        o = make_opts(**opts)
+        st = BatchStats()
+        core = np.zeros(max(self.vertices, 1), np.uint32) if copy else None
+        km = np.zeros(max(self.G, 1), np.uint32) if copy else None
+        _check(lib().kc_batch_run_plain(self._h, m, ctypes.byref(o), _ptr(core), _ptr(km),
+                                        ctypes.byref(st)))
+        if not copy:
+            return None, None, st.as_dict()
+        return core[:self.vertices], km[:self.G], st.as_dict()
+
+    def download(self):
+        h = _vp()
+        _check(lib().kc_batch_download(self._h, ctypes.byref(h)))
+        return HostBatch(h)
+
+    def autotune(self, reps=1):
+        o = BatchOpts()
+        ms = ctypes.c_double(0)
+        _check(lib().kc_batch_autotune(self._h, reps, ctypes.byref(o), ctypes.byref(ms)))
+        return {"block": o.block, "per_cu": o.per_cu}, ms.value
+
+    def close(self):
+        if self._h:
+            lib().kc_batch_free(self._h)
             self._h = None
 
     def __del__(self):
