@@ -9,11 +9,17 @@ same graph and reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
+Beside it, the batched workload: 4,096 R-MAT graphs (scale uniform over the
+integers 8..18, edge factor uniform in [4, 32], seed base+g) decomposed in
+one persistent launch, against per-graph plain launches of the same engine
+on the same resident data (graphs/s both ways, every graph checked against
+the oracle).
+
 Prints one JSON line.  value = GTEPS = undirected edges / seconds per step.
 roofline.achieved = algorithmic bytes per launch / mean kernel time, with
 the byte model
     B_peel = 4(V+1) + 4*2m (col) + 4*2m (degree RMW) + 4V + 4V(kmax+1)
-    B_h    = 4(V+1) + 12*2m + 4(2m+V) + 4V + rounds*V + 4*changed + 16*deg(changed)
+    B_h    = 4(V+1) + 12*2m + 4(2m+V) + 4V + rounds*V + 4*changed + 16*sum deg(changed)
 (2m = adjacency entries).  cpu_baseline = the oracle's sequential
 Batagelj-Zaversnik peel on one host core over the same graph.
 """
@@ -42,6 +48,9 @@ def parse():
     ap.add_argument("--hindex-method", default="histocore")
     ap.add_argument("--cpu-runs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true")
     return ap.parse_args()
 
 
@@ -77,6 +86,67 @@ def traffic_from_profile(kernel_key):
         return None
 
 
+def bench_batched(ctx, args):
+    """4,096 uneven R-MAT graphs: one persistent launch vs per-graph plain
+    launches, both on the same HBM-resident batch."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import oracle
+
+    G = args.batch_graphs
+    t0 = time.time()
+    db = ctx.gen_rmat_batch(G, 8, 18, 4.0, 32.0, seed=args.seed)
+    gen_s = time.time() - t0
+    best, tune_ms = db.autotune(reps=1)
+    for _ in range(max(1, args.warmup)):
+        db.run(copy=False)
+    wall, dev = [], []
+    for _ in range(args.batch_steps):
+        _, _, st = db.run(copy=False)
+        wall.append(st["total_ms"])
+        dev.append(st["device_ms"])
+    core, km, st = db.run()
+    _, _, sp = db.run_plain("peelone_dyn", copy=False)
+    # every graph against the oracle's sequential peel, on the host's cores
+    hb = db.download()
+    vo, eo, rp, cl = hb.arrays()
+
+    def check(gi):
+        V = int(vo[gi + 1] - vo[gi])
+        want = oracle.bz(rp[vo[gi] + gi: vo[gi] + gi + V + 1].astype(np.uint64),
+                         cl[eo[gi]: eo[gi + 1]])
+        return bool(np.array_equal(core[vo[gi]: vo[gi] + V], want)
+                    and km[gi] == (int(want.max()) if V else 0))
+
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        ok = sum(ex.map(check, range(G)))
+    cpu_s = time.perf_counter() - t
+    ms = float(np.median(wall))
+    out = {
+        "workload": "%d R-MAT graphs (.57,.19,.19,.05), scale uniform over the integers 8..18, "
+                    "edge factor uniform in [4,32], seed %d+g; concatenated int32 CSR in HBM"
+                    % (G, args.seed),
+        "vertices": int(hb.vertices), "adjacency_entries": int(hb.entries),
+        "graphs_per_s": round(G / (ms * 1e-3), 1),
+        "ms_per_batch": round(ms, 3), "device_ms": round(float(np.median(dev)), 3),
+        "launches": 1, "steps": args.batch_steps,
+        "plain_graphs_per_s": round(G / (sp["total_ms"] * 1e-3), 1),
+        "plain_ms": round(sp["total_ms"], 1), "plain_device_ms": round(sp["device_ms"], 1),
+        "plain_launches": sp["launches"], "plain_method": "peelone_dyn",
+        "speedup_vs_plain": round(sp["total_ms"] / ms, 2),
+        "autotuned": best, "levels_run": st["levels_run"], "kmax": st["kmax"],
+        "parity_graphs": "%d/%d" % (ok, G),
+        "cpu_bz_s": round(cpu_s, 2), "cpu_bz_threads": threads,
+        "gen_s": round(gen_s, 2),
+    }
+    db.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -88,13 +158,15 @@ def main():
 
     import pico_amd as pa
 
+    ctx = pa.Context(0)
     t0 = time.time()
-    g = pa.HostGraph.rmat(args.scale, args.edge_factor, seed=args.seed)
+    # built in HBM (the host generator's graph, edge for edge); resident
+    # before timing starts
+    dg = ctx.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
     gen_s = time.time() - t0
+    g = dg.download()  # host copy for the oracle and the CPU baseline
     V, nnz = g.V, g.nnz
     m = nnz // 2
-    ctx = pa.Context(0)
-    dg = ctx.upload(g)  # the graph is resident in HBM before timing starts
 
     # ---------------------------------------------------- plain peel steps
     for _ in range(args.warmup):
@@ -121,7 +193,7 @@ def main():
         h_ms.append(hst["device_ms"])
     hcore, hst = dg.decompose(args.hindex_method)
     h_kernel_ms = float(np.mean(h_ms))
-    bh = hindex_bytes(V, nnz, hst["iterations"], hst["changed"], hst["edges_walked"])
+    bh = hindex_bytes(V, nnz, hst["iterations"], hst["changed"], hst["changed_degree"])
 
     # ------------------------------------------------------- CPU baseline
     cpu = None
@@ -141,6 +213,8 @@ def main():
                          "(oracle/kc_oracle.c), mean of %d runs = %.3f s; host %s"
                          % (len(times), cpu_s, cpu_model())}
         parity = bool(np.array_equal(core, ref) and np.array_equal(hcore, ref))
+
+    batched = None if args.no_batch else bench_batched(ctx, args)
 
     out = {
         "metric": "k-core GTEPS + % HBM roofline (R-MAT s22); batched vs plain graphs/s; p99",
@@ -177,6 +251,7 @@ def main():
                                 "kernel": "hindex_kernel", "algorithmic_bytes": bh}},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     dg.close()

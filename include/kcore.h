@@ -93,7 +93,9 @@ typedef struct kc_stats {
     uint64_t changed; /* h-index: core-value changes over all rounds */
     double prep_ms;   /* one-time layout preparation done by this call (HistoCore's
                          degree-ordered adjacency), outside device_ms */
-    uint64_t reserved[2];
+    uint64_t changed_degree; /* HistoCore: degrees of the changed vertices summed over
+                                rounds (the adjacency the reference's update walks) */
+    uint64_t reserved[1];
 } kc_stats;
 
 KC_API const char *kc_last_error(void);

@@ -24,6 +24,8 @@
 
 namespace kc {
 
+static constexpr int BU = 4; // cascade edges per thread per pass
+
 template <int BLOCK, int LDS_V> struct BLds {
     uint32_t deg[LDS_V];
     uint64_t start[BLOCK];
@@ -204,26 +206,42 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchArgs a)
                 if (threadIdx.x == 0)
                     L.head = h + m;
                 const uint64_t total = L.pre[m];
-                for (uint64_t e0 = 0; e0 < total; e0 += BLOCK) {
-                    uint64_t e = e0 + threadIdx.x;
-                    bool claim = false;
-                    uint32_t u = 0;
-                    if (e < total) {
-                        uint32_t i = b_find(L.pre, m, e);
-                        u = col[L.start[i] + (e - L.pre[i])];
-                        if (deg[u] > level) {
-                            uint32_t old = in_lds ? atomicSub(&deg[u], 1u) : wg_sub(&deg[u], 1u);
+                // BU edges per thread per pass, their loads and atomics
+                // independent so they are in flight together
+                for (uint64_t e0 = 0; e0 < total; e0 += (uint64_t)BLOCK * BU) {
+                    uint32_t u[BU], d[BU];
+                    bool claim[BU];
+#pragma unroll
+                    for (int j = 0; j < BU; j++) {
+                        uint64_t e = e0 + (uint64_t)j * BLOCK + threadIdx.x;
+                        u[j] = 0xffffffffu;
+                        if (e < total) {
+                            uint32_t i = b_find(L.pre, m, e);
+                            u[j] = col[L.start[i] + (e - L.pre[i])];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < BU; j++)
+                        d[j] = u[j] != 0xffffffffu ? deg[u[j]] : 0u;
+#pragma unroll
+                    for (int j = 0; j < BU; j++) {
+                        claim[j] = false;
+                        if (d[j] > level) {
+                            uint32_t old = in_lds ? atomicSub(&deg[u[j]], 1u)
+                                                  : wg_sub(&deg[u[j]], 1u);
                             if (old == level + 1)
-                                claim = true;
+                                claim[j] = true;
                             else if (old <= level) {
                                 if (in_lds)
-                                    atomicAdd(&deg[u], 1u);
+                                    atomicAdd(&deg[u[j]], 1u);
                                 else
-                                    wg_add(&deg[u], 1u);
+                                    wg_add(&deg[u[j]], 1u);
                             }
                         }
                     }
-                    b_push<BLOCK, LDS_V>(L, q, claim, u);
+#pragma unroll
+                    for (int j = 0; j < BU; j++)
+                        b_push<BLOCK, LDS_V>(L, q, claim[j], u[j]);
                 }
                 __syncthreads();
             }

@@ -668,6 +668,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     st.phases = sv[kc::KC_ST_PHASES];
     st.edges_walked = sv[kc::KC_ST_EDGES];
     st.changed = sv[kc::KC_ST_CHANGED];
+    st.changed_degree = sv[kc::KC_ST_CHG_DEG];
     if (p.trace)
         g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     st.grid = grid;

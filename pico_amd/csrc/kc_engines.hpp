@@ -20,6 +20,7 @@ enum {
     KC_ST_EDGES = 4,      // adjacency entries walked
     KC_ST_CHANGED = 5,    // h-index: total core-value changes
     KC_ST_ERR = 6,
+    KC_ST_CHG_DEG = 7,    // h-index: degrees of changed vertices, summed
     KC_ST_COUNT = 8
 };
 

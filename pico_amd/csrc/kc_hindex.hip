@@ -388,6 +388,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     uint32_t round = 0;
     uint64_t changed_total = 0;
     uint64_t upd_edges = 0; // adjacency entries of changed vertices walked (HistoCore)
+    uint64_t chg_deg = 0;   // degrees of changed vertices, summed (HistoCore)
     uint32_t cap_list = V ? V : 1;
     uint32_t *act_cur = a.list[0]; // CntCore active lists, swapped per round
     uint32_t *act_nxt = a.list[3];
@@ -469,6 +470,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     a.core[v] = k;
                     L.stv[wv][n_st] = v;
                     L.stn[wv][n_st] = nch;
+                    chg_deg += d;
                 }
                 n_st++;
                 st_nch += nch;
@@ -521,6 +523,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                             a.histo[b] = old;
                                             a.histo[b + k] = sum;
                                             a.core[v] = k;
+                                            chg_deg += d;
                                             atomicAdd((unsigned long long *)&gs->slot[phase & 1].sum0,
                                                       1ull);
                                             L.bk = atomicAdd(&gs->slot[phase & 1].qtail, nch);
@@ -785,6 +788,9 @@ out:
     uint64_t ue = wave_sum_u64(upd_edges);
     if (lane == 0 && ue)
         atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], (unsigned long long)ue);
+    uint64_t cd = wave_sum_u64(chg_deg);
+    if (lane == 0 && cd)
+        atomicAdd((unsigned long long *)&a.stats[KC_ST_CHG_DEG], (unsigned long long)cd);
 }
 
 template <typename R>

@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
                 ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
-                ("prep_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 2)]
+                ("prep_ms", ctypes.c_double), ("changed_degree", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 1)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_
