@@ -21,6 +21,8 @@
  *  - kc_peel_dyn / kc_peelone_dyn   <- `int peel_dyn(Graph&)` (peel-dyn.cc:133) and
  *    `int peelone_dyn(Graph&)` (peelone-dyn.cc:120), the function pointers
  *    main.cu:28-29 hands to repSimulation
+ *  - kc_update                       <- no counterpart: core maintenance under
+ *    edge insertions/deletions is new functionality
  *  - kc_batch_*, kc_hb_*, kc_autotune  <- no counterpart: the reference runs
  *    one graph per process with the fixed launch constants of
  *    peel/inc/common.h:4-14 and peel/shared/argument.h:24
@@ -308,6 +310,38 @@ KC_API kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min,
 KC_API kc_status kc_batch_download(const kc_batch *b, kc_host_batch **out);
 KC_API kc_status kc_batch_dims(const kc_batch *b, uint32_t *G, uint64_t *vertices,
                                uint64_t *entries);
+
+/* ------------------------------------------------------- edge updates */
+/* Core maintenance under batched edge insertions and deletions (no
+ * counterpart in the reference, whose graphs are static).  The first call
+ * moves the graph into a slack layout on the device and peels it once; each
+ * call then applies the batch and leaves the exact core numbers of the
+ * updated graph on the device (core_out, V entries, may be NULL).  Pairs are
+ * unordered; self-loops, repeated pairs, inserts of present edges and
+ * deletes of absent edges are skipped and counted.  kc_decompose and
+ * kc_graph_download see the updated graph. */
+typedef enum kc_update_op { KC_INSERT = 0, KC_DELETE = 1 } kc_update_op;
+
+typedef struct kc_update_stats {
+    uint64_t requested;   /* pairs passed in */
+    uint64_t applied;     /* edges inserted / deleted */
+    uint64_t skipped;     /* requested - applied */
+    uint64_t candidates;  /* vertices raised by one before lowering (inserts) */
+    uint64_t evaluations; /* worklist entries evaluated while lowering */
+    uint64_t changed;     /* vertices whose core number changed */
+    uint32_t matchings;   /* vertex-disjoint sub-batches applied in turn */
+    uint32_t bfs_levels;  /* candidate-search levels (inserts) */
+    uint32_t rounds;      /* lowering rounds */
+    uint32_t grows;       /* row re-layouts for lack of spare slots */
+    double device_ms;     /* HIP-event time from first to last device step */
+    double total_ms;      /* host wall time of the call */
+    uint64_t reserved[4];
+} kc_update_stats;
+
+KC_API kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u, const uint32_t *v,
+                           uint64_t n, uint32_t *core_out, kc_update_stats *stats);
+/* current vertex count and adjacency entries (2x undirected edges) */
+KC_API kc_status kc_graph_dims(const kc_graph *g, uint32_t *V, uint64_t *nnz);
 
 /* ------------------------------------------------------------ autotuner */
 /* Time the launch candidates of `method` on graph g (block size, virtual

@@ -32,8 +32,10 @@ LIB_SOURCES = [
     os.path.join(CSRC, "kc_prep.hip"),
     os.path.join(CSRC, "kc_batch.hip"),
     os.path.join(CSRC, "kc_gen.hip"),
+    os.path.join(CSRC, "kc_dyn.hip"),
     os.path.join(CSRC, "kc_capi.cpp"),
     os.path.join(CSRC, "kc_batch_capi.cpp"),
+    os.path.join(CSRC, "kc_dyn_capi.cpp"),
     os.path.join(HOST, "kc_host.cpp"),
 ]
 

@@ -461,6 +461,11 @@ kc_status kc_graph_download(const kc_graph *g, kc_host_graph **out)
 {
     if (!g || !out)
         return fail(KC_EINVAL, "null argument");
+    if (g->csr_stale) { // edge updates since the CSR was last laid out
+        kc_status s = dyn_materialize(const_cast<kc_graph *>(g));
+        if (s)
+            return s;
+    }
     auto *h = new (std::nothrow) kc_host_graph;
     if (!h)
         return fail(KC_ENOMEM, "host allocation");

@@ -399,6 +399,7 @@ void kc_graph_free(kc_graph *g)
         hipEventDestroy(g->ev0);
     if (g->ev1)
         hipEventDestroy(g->ev1);
+    dyn_free(g->dyn);
     delete g;
 }
 
@@ -679,6 +680,11 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
 kc_status kci::run_method(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    if (g->csr_stale) {
+        kc_status s = dyn_materialize(g);
+        if (s)
+            return s;
+    }
     switch (method) {
     case KC_GPP:
     case KC_PEELONE:

@@ -40,6 +40,9 @@ struct DevBuf {
     size_t n = 0;
 };
 
+struct DynState; // slack CSR + maintained cores (kc_dyn_capi.cpp)
+void dyn_free(DynState *d);
+
 struct kc_graph {
     kc_ctx *ctx = nullptr;
     uint32_t V = 0;
@@ -57,6 +60,8 @@ struct kc_graph {
     const uint32_t *result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     kc_opts tuned[7] = {}; // autotuner's pick per method (block == 0: none)
+    DynState *dyn = nullptr; // created by the first kc_update
+    bool csr_stale = false;  // rowptr/col predate the last kc_update
 };
 
 // fill the zero fields of o from the graph's tuned options for `method`
@@ -81,4 +86,6 @@ kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what);
 kc_status run_method(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st);
 // the process-wide context on device 0 used by the reference-shaped entries
 kc_ctx *default_ctx();
+// rebuild rowptr/col from the slack CSR after edge updates
+kc_status dyn_materialize(kc_graph *g);
 } // namespace kci

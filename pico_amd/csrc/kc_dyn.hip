@@ -1,0 +1,371 @@
+// kc_dyn.hip -- core maintenance under batched edge insertions and deletions.
+//
+// The graph lives in a slack CSR: vertex v owns adjacency slots
+// [off[v], off[v+1]) of which the first len[v] are its current neighbours
+// (unordered).  Insertions append, deletions swap the last neighbour into
+// the freed slot; one launch only ever touches a matching, so no two waves
+// edit the same row.
+//
+// Exactness rests on two facts about c <- min(c, H(c)), the h-index operator
+// of HistoCore's count_kernel (HistoCore.cu:77-128):
+//   * from any upper bound of the core vector, lowering a vertex by one
+//     whenever fewer than c(v) neighbours have c >= c(v) stays an upper bound
+//     and stops exactly at the core vector (a fixed point c <= H(c) is a
+//     lower bound, since {c >= k} then has minimum degree >= k);
+//   * inserting a matching raises every core by at most one, and only the
+//     vertices connected to a root (the endpoint(s) of an inserted edge
+//     holding the edge's smaller core K) through vertices of core K that
+//     have more than K neighbours of core >= K can rise.
+// Deletions start from the old cores (an upper bound) at the roots;
+// insertions raise the candidate region by one, then lower it.
+#include "kc_device.hpp"
+#include "kc_engines.hpp"
+
+namespace kc {
+
+__device__ __forceinline__ uint32_t wave_id_global()
+{
+    return (blockIdx.x * blockDim.x + threadIdx.x) / KC_WAVE;
+}
+__device__ __forceinline__ uint32_t waves_total() { return gridDim.x * blockDim.x / KC_WAVE; }
+
+// position of `x` in the first n entries of row[], or -1 (whole wave)
+__device__ __forceinline__ int64_t wave_find(const uint32_t *row, uint32_t n, uint32_t x)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t i0 = 0; i0 < n; i0 += KC_WAVE) {
+        uint32_t i = i0 + lane;
+        uint64_t m = ballot(i < n && row[i] == x);
+        if (m)
+            return (int64_t)i0 + (__ffsll((unsigned long long)m) - 1);
+    }
+    return -1;
+}
+
+// neighbours of w with core >= k, counting stops once `enough` is reached
+__device__ __forceinline__ uint32_t wave_count_ge(const DynView &d, uint32_t w, uint32_t k,
+                                                  uint32_t enough)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t *row = d.adj + d.off[w];
+    const uint32_t n = d.len[w];
+    uint32_t cnt = 0;
+    for (uint32_t i0 = 0; i0 < n && cnt < enough; i0 += KC_WAVE) {
+        uint32_t i = i0 + lane;
+        bool ge = i < n && ld_relaxed(&d.core[row[i]]) >= k;
+        cnt += (uint32_t)__popcll(ballot(ge));
+    }
+    return cnt;
+}
+
+// wave-aggregated push of the lanes' `want` vertices; vertices already
+// carrying `stamp` in mark[] are dropped (each vertex enters a list once)
+__device__ __forceinline__ void wave_push_claim(bool want, uint32_t v, uint32_t *mark,
+                                                uint32_t stamp, uint32_t *list, uint32_t *n_list,
+                                                uint32_t cap)
+{
+    if (want)
+        want = atomicExch(&mark[v], stamp) != stamp;
+    uint64_t m = ballot(want);
+    if (!m)
+        return;
+    const uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(n_list, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, KC_WAVE);
+    uint32_t pos = base + lane_rank(m);
+    if (want && pos < cap)
+        list[pos] = v;
+}
+
+// ---------------------------------------------------------- edge edits
+// status[i]: 0 not applied (present for an insert, absent for a delete),
+// 1 applied, 2 not applied for lack of row capacity
+__global__ void dyn_edit_kernel(DynView d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                                uint32_t op, uint8_t *status)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t i = wave_id_global(); i < n; i += waves_total()) {
+        uint32_t x = eu[i], y = ev[i];
+        uint32_t *rx = d.adj + d.off[x], *ry = d.adj + d.off[y];
+        uint32_t lx = d.len[x], ly = d.len[y];
+        int64_t px = wave_find(rx, lx, y);
+        uint8_t st = 0;
+        if (op == 0) { // insert
+            if (px < 0) {
+                uint64_t capx = d.off[x + 1] - d.off[x], capy = d.off[y + 1] - d.off[y];
+                if (lx < capx && ly < capy) {
+                    if (lane == 0) {
+                        rx[lx] = y;
+                        ry[ly] = x;
+                        d.len[x] = lx + 1;
+                        d.len[y] = ly + 1;
+                    }
+                    st = 1;
+                } else {
+                    st = 2;
+                }
+            }
+        } else if (px >= 0) { // delete
+            int64_t py = wave_find(ry, ly, x);
+            if (lane == 0) {
+                rx[px] = rx[lx - 1];
+                d.len[x] = lx - 1;
+                if (py >= 0) {
+                    ry[py] = ry[ly - 1];
+                    d.len[y] = ly - 1;
+                }
+            }
+            st = 1;
+        }
+        if (lane == 0)
+            status[i] = st;
+    }
+}
+
+// roots of the applied edges: endpoint(s) holding the edge's smaller core
+__global__ void dyn_roots_kernel(DynView d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                                 const uint8_t *status, uint32_t *mark, uint32_t stamp,
+                                 uint32_t *list, uint32_t *n_list, uint32_t cap)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = i < n && status[i] == 1;
+    uint32_t x = live ? eu[i] : 0, y = live ? ev[i] : 0;
+    uint32_t cx = live ? d.core[x] : 0, cy = live ? d.core[y] : 0;
+    wave_push_claim(live && cx <= cy, x, mark, stamp, list, n_list, cap);
+    wave_push_claim(live && cy <= cx, y, mark, stamp, list, n_list, cap);
+}
+
+// One level of the candidate search: w (core K) is a candidate when more
+// than K neighbours have core >= K; candidates extend the search to their
+// unvisited neighbours of core K.
+__global__ void dyn_bfs_kernel(DynView d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                               uint32_t *n_out, uint32_t *visit, uint32_t stamp, uint32_t *cand,
+                               uint32_t *n_cand, uint32_t cap)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t j = wave_id_global(); j < n_in; j += waves_total()) {
+        uint32_t w = in[j];
+        uint32_t K = d.core[w];
+        if (wave_count_ge(d, w, K, K + 1) <= K)
+            continue;
+        if (lane == 0) {
+            uint32_t p = atomicAdd(n_cand, 1u);
+            if (p < cap)
+                cand[p] = w;
+        }
+        const uint32_t *row = d.adj + d.off[w];
+        const uint32_t n = d.len[w];
+        for (uint32_t i0 = 0; i0 < n; i0 += KC_WAVE) {
+            uint32_t i = i0 + lane;
+            uint32_t u = i < n ? row[i] : 0;
+            bool want = i < n && d.core[u] == K;
+            wave_push_claim(want, u, visit, stamp, out, n_out, cap);
+        }
+    }
+}
+
+__global__ void dyn_raise_kernel(uint32_t *core, const uint32_t *cand, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        core[cand[i]] += 1;
+}
+
+// One round of the lowering iteration over a worklist: w drops by one when
+// fewer than c(w) neighbours have core >= c(w); it then re-enters the next
+// round itself, with its neighbours of core c(w) (their support at c(w)
+// fell).
+__global__ void dyn_round_kernel(DynView d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                                 uint32_t *n_out, uint32_t *mark, uint32_t stamp,
+                                 uint32_t *n_drops, uint32_t cap)
+{
+    const uint32_t lane = lane_id();
+    uint32_t drops = 0;
+    for (uint32_t j = wave_id_global(); j < n_in; j += waves_total()) {
+        uint32_t w = in[j];
+        uint32_t c = ld_relaxed(&d.core[w]);
+        if (c == 0 || wave_count_ge(d, w, c, c) >= c)
+            continue;
+        if (lane == 0) {
+            st_relaxed(&d.core[w], c - 1);
+            drops++;
+        }
+        wave_push_claim(lane == 0, w, mark, stamp, out, n_out, cap);
+        const uint32_t *row = d.adj + d.off[w];
+        const uint32_t n = d.len[w];
+        for (uint32_t i0 = 0; i0 < n; i0 += KC_WAVE) {
+            uint32_t i = i0 + lane;
+            uint32_t u = i < n ? row[i] : 0;
+            bool want = i < n && ld_relaxed(&d.core[u]) == c;
+            wave_push_claim(want, u, mark, stamp, out, n_out, cap);
+        }
+    }
+    if (lane == 0 && drops)
+        atomicAdd(n_drops, drops);
+}
+
+// ------------------------------------------------------- layout changes
+// copy rows into a new slack layout (wave per vertex)
+__global__ void dyn_copy_rows(const uint64_t *off_src, const uint32_t *len, const uint32_t *src,
+                              const uint64_t *off_dst, uint32_t *dst, uint32_t V)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t v = wave_id_global(); v < V; v += waves_total()) {
+        const uint32_t *s = src + off_src[v];
+        uint32_t *t = dst + off_dst[v];
+        uint32_t n = len[v];
+        for (uint32_t i = lane; i < n; i += KC_WAVE)
+            t[i] = s[i];
+    }
+}
+
+template <typename R>
+__global__ void dyn_rows_from_csr(const R *rowptr, const uint32_t *col, const uint64_t *off,
+                                  uint32_t *len, uint32_t *adj, uint32_t V)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t v = wave_id_global(); v < V; v += waves_total()) {
+        uint64_t s = rowptr[v], e = rowptr[v + 1];
+        uint32_t *t = adj + off[v];
+        for (uint64_t i = s + lane; i < e; i += KC_WAVE)
+            t[i - s] = col[i];
+        if (lane == 0)
+            len[v] = (uint32_t)(e - s);
+    }
+}
+
+template <typename R>
+__global__ void dyn_rows_to_csr(const uint64_t *off, const uint32_t *len, const uint32_t *adj,
+                                const R *rowptr, uint32_t *col, uint32_t V)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t v = wave_id_global(); v < V; v += waves_total()) {
+        const uint32_t *s = adj + off[v];
+        uint64_t b = rowptr[v];
+        uint32_t n = len[v];
+        for (uint32_t i = lane; i < n; i += KC_WAVE)
+            col[b + i] = s[i];
+    }
+}
+
+__global__ void dyn_count_changed(const uint32_t *a, const uint32_t *b, uint32_t V, uint32_t *n)
+{
+    uint32_t c = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        c += a[v] != b[v];
+    c = (uint32_t)wave_sum_u64(c);
+    if (lane_id() == 0 && c)
+        atomicAdd(n, c);
+}
+
+// ----------------------------------------------------------- launchers
+static constexpr uint32_t DYN_BLOCK = 256;
+static inline dim3 wave_grid(uint64_t items)
+{
+    uint64_t waves = items ? items : 1;
+    uint64_t blocks = (waves * KC_WAVE + DYN_BLOCK - 1) / DYN_BLOCK;
+    return dim3((uint32_t)(blocks < 8192 ? blocks : 8192));
+}
+static inline dim3 thread_grid(uint64_t items)
+{
+    uint64_t blocks = (items + DYN_BLOCK - 1) / DYN_BLOCK;
+    blocks = blocks ? blocks : 1;
+    return dim3((uint32_t)(blocks < 8192 ? blocks : 8192));
+}
+
+hipError_t dyn_edit(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                    uint32_t op, uint8_t *status, hipStream_t st)
+{
+    if (n)
+        hipLaunchKernelGGL(dyn_edit_kernel, wave_grid(n), dim3(DYN_BLOCK), 0, st, d, eu, ev, n,
+                           op, status);
+    return hipGetLastError();
+}
+
+hipError_t dyn_roots(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                     const uint8_t *status, uint32_t *mark, uint32_t stamp, uint32_t *list,
+                     uint32_t *n_list, uint32_t cap, hipStream_t st)
+{
+    if (n)
+        hipLaunchKernelGGL(dyn_roots_kernel, dim3((n + DYN_BLOCK - 1) / DYN_BLOCK),
+                           dim3(DYN_BLOCK), 0, st, d, eu, ev, n, status, mark, stamp, list,
+                           n_list, cap);
+    return hipGetLastError();
+}
+
+hipError_t dyn_bfs(const DynView &d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                   uint32_t *n_out, uint32_t *visit, uint32_t stamp, uint32_t *cand,
+                   uint32_t *n_cand, uint32_t cap, hipStream_t st)
+{
+    if (n_in)
+        hipLaunchKernelGGL(dyn_bfs_kernel, wave_grid(n_in), dim3(DYN_BLOCK), 0, st, d, in, n_in,
+                           out, n_out, visit, stamp, cand, n_cand, cap);
+    return hipGetLastError();
+}
+
+hipError_t dyn_raise(uint32_t *core, const uint32_t *cand, uint32_t n, hipStream_t st)
+{
+    if (n)
+        hipLaunchKernelGGL(dyn_raise_kernel, thread_grid(n), dim3(DYN_BLOCK), 0, st, core, cand,
+                           n);
+    return hipGetLastError();
+}
+
+hipError_t dyn_round(const DynView &d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                     uint32_t *n_out, uint32_t *mark, uint32_t stamp, uint32_t *n_drops,
+                     uint32_t cap, hipStream_t st)
+{
+    if (n_in)
+        hipLaunchKernelGGL(dyn_round_kernel, wave_grid(n_in), dim3(DYN_BLOCK), 0, st, d, in,
+                           n_in, out, n_out, mark, stamp, n_drops, cap);
+    return hipGetLastError();
+}
+
+hipError_t dyn_copy(const uint64_t *off_src, const uint32_t *len, const uint32_t *src,
+                    const uint64_t *off_dst, uint32_t *dst, uint32_t V, hipStream_t st)
+{
+    if (V)
+        hipLaunchKernelGGL(dyn_copy_rows, wave_grid(V), dim3(DYN_BLOCK), 0, st, off_src, len,
+                           src, off_dst, dst, V);
+    return hipGetLastError();
+}
+
+hipError_t dyn_from_csr(const void *rowptr, bool wide, const uint32_t *col, const uint64_t *off,
+                        uint32_t *len, uint32_t *adj, uint32_t V, hipStream_t st)
+{
+    if (!V)
+        return hipSuccess;
+    if (wide)
+        hipLaunchKernelGGL(dyn_rows_from_csr<uint64_t>, wave_grid(V), dim3(DYN_BLOCK), 0, st,
+                           (const uint64_t *)rowptr, col, off, len, adj, V);
+    else
+        hipLaunchKernelGGL(dyn_rows_from_csr<uint32_t>, wave_grid(V), dim3(DYN_BLOCK), 0, st,
+                           (const uint32_t *)rowptr, col, off, len, adj, V);
+    return hipGetLastError();
+}
+
+hipError_t dyn_to_csr(const uint64_t *off, const uint32_t *len, const uint32_t *adj,
+                      const void *rowptr, bool wide, uint32_t *col, uint32_t V, hipStream_t st)
+{
+    if (!V)
+        return hipSuccess;
+    if (wide)
+        hipLaunchKernelGGL(dyn_rows_to_csr<uint64_t>, wave_grid(V), dim3(DYN_BLOCK), 0, st, off,
+                           len, adj, (const uint64_t *)rowptr, col, V);
+    else
+        hipLaunchKernelGGL(dyn_rows_to_csr<uint32_t>, wave_grid(V), dim3(DYN_BLOCK), 0, st, off,
+                           len, adj, (const uint32_t *)rowptr, col, V);
+    return hipGetLastError();
+}
+
+hipError_t dyn_changed(const uint32_t *a, const uint32_t *b, uint32_t V, uint32_t *n,
+                       hipStream_t st)
+{
+    if (V)
+        hipLaunchKernelGGL(dyn_count_changed, thread_grid(V), dim3(DYN_BLOCK), 0, st, a, b, V, n);
+    return hipGetLastError();
+}
+
+} // namespace kc

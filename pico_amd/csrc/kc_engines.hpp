@@ -127,6 +127,37 @@ struct BuildOut {
 };
 hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std::string &err);
 
+// Core maintenance under edge updates (kc_dyn.hip) on a slack CSR: vertex v
+// owns slots [off[v], off[v+1]) of adj, the first len[v] of them in use.
+struct DynView {
+    uint32_t V;
+    const uint64_t *off; // V+1
+    uint32_t *len;       // V
+    uint32_t *adj;
+    uint32_t *core; // V, maintained core numbers
+};
+// op 0 insert / 1 delete of a matching; status[i]: 0 skipped, 1 applied, 2 row full
+hipError_t dyn_edit(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                    uint32_t op, uint8_t *status, hipStream_t st);
+hipError_t dyn_roots(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
+                     const uint8_t *status, uint32_t *mark, uint32_t stamp, uint32_t *list,
+                     uint32_t *n_list, uint32_t cap, hipStream_t st);
+hipError_t dyn_bfs(const DynView &d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                   uint32_t *n_out, uint32_t *visit, uint32_t stamp, uint32_t *cand,
+                   uint32_t *n_cand, uint32_t cap, hipStream_t st);
+hipError_t dyn_raise(uint32_t *core, const uint32_t *cand, uint32_t n, hipStream_t st);
+hipError_t dyn_round(const DynView &d, const uint32_t *in, uint32_t n_in, uint32_t *out,
+                     uint32_t *n_out, uint32_t *mark, uint32_t stamp, uint32_t *n_drops,
+                     uint32_t cap, hipStream_t st);
+hipError_t dyn_copy(const uint64_t *off_src, const uint32_t *len, const uint32_t *src,
+                    const uint64_t *off_dst, uint32_t *dst, uint32_t V, hipStream_t st);
+hipError_t dyn_from_csr(const void *rowptr, bool wide, const uint32_t *col, const uint64_t *off,
+                        uint32_t *len, uint32_t *adj, uint32_t V, hipStream_t st);
+hipError_t dyn_to_csr(const uint64_t *off, const uint32_t *len, const uint32_t *adj,
+                      const void *rowptr, bool wide, uint32_t *col, uint32_t V, hipStream_t st);
+hipError_t dyn_changed(const uint32_t *a, const uint32_t *b, uint32_t V, uint32_t *n,
+                       hipStream_t st);
+
 hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st);
 int batch_occupancy(uint32_t block);
 uint32_t batch_lds_vertices(uint32_t block);

@@ -100,6 +100,20 @@ class BatchStats(ctypes.Structure):
                 if not k.startswith("reserved")}
 
 
+class UpdateStats(ctypes.Structure):
+    _fields_ = [("requested", ctypes.c_uint64), ("applied", ctypes.c_uint64),
+                ("skipped", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
+                ("evaluations", ctypes.c_uint64), ("changed", ctypes.c_uint64),
+                ("matchings", ctypes.c_uint32), ("bfs_levels", ctypes.c_uint32),
+                ("rounds", ctypes.c_uint32), ("grows", ctypes.c_uint32),
+                ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if not k.startswith("reserved")}
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -179,6 +193,9 @@ def lib():
                                ctypes.POINTER(_vp)], S),
         "kc_batch_download": ([_vp, ctypes.POINTER(_vp)], S),
         "kc_batch_dims": ([_vp, _u32p, _u64p, _u64p], S),
+        "kc_update": ([_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp,
+                       ctypes.POINTER(UpdateStats)], S),
+        "kc_graph_dims": ([_vp, _u32p, _u64p], S),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -447,6 +464,26 @@ class DeviceGraph:
         h = _vp()
         _check(lib().kc_graph_download(self._h, ctypes.byref(h)))
         return HostGraph(h)
+
+    @property
+    def nnz(self):
+        n = ctypes.c_uint64(0)
+        _check(lib().kc_graph_dims(self._h, None, ctypes.byref(n)))
+        return n.value
+
+    def update(self, op, u, v, copy=True):
+        """Insert (op "insert") or delete ("delete") the unordered pairs
+        (u[i], v[i]) and return (maintained cores, stats dict)."""
+        o = {"insert": 0, "delete": 1}[op] if isinstance(op, str) else int(op)
+        uu = np.ascontiguousarray(u, np.uint32)
+        vv = np.ascontiguousarray(v, np.uint32)
+        if uu.size != vv.size:
+            raise ValueError("u and v differ in length")
+        core = np.zeros(max(self.V, 1), np.uint32) if copy else None
+        st = UpdateStats()
+        _check(lib().kc_update(self._h, o, _ptr(uu), _ptr(vv), uu.size, _ptr(core),
+                               ctypes.byref(st)))
+        return (core[:self.V] if copy else None), st.as_dict()
 
     @property
     def device_bytes(self):
