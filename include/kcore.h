@@ -21,6 +21,8 @@
  *  - kc_peel_dyn / kc_peelone_dyn   <- `int peel_dyn(Graph&)` (peel-dyn.cc:133) and
  *    `int peelone_dyn(Graph&)` (peelone-dyn.cc:120), the function pointers
  *    main.cu:28-29 hands to repSimulation
+ *  - kc_server_*                     <- no counterpart: the reference has no
+ *    request loop
  *  - kc_update                       <- no counterpart: core maintenance under
  *    edge insertions/deletions is new functionality
  *  - kc_batch_*, kc_hb_*, kc_autotune  <- no counterpart: the reference runs
@@ -308,6 +310,14 @@ KC_API kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min,
                                    uint32_t scale_max, double ef_min, double ef_max, double a,
                                    double b, double c, uint64_t seed, kc_batch **out);
 KC_API kc_status kc_batch_download(const kc_batch *b, kc_host_batch **out);
+/* graphs with explicit (scale, generated edges, seed) each */
+KC_API kc_status kc_batch_gen_rmat_list(kc_ctx *ctx, uint32_t G, const uint32_t *scale,
+                                        const uint64_t *m_gen, const uint64_t *seeds, double a,
+                                        double b, double c, kc_batch **out);
+/* per-graph kc_core_checksum of the last kc_batch_run's cores (G entries) */
+KC_API kc_status kc_batch_checksums(kc_batch *b, uint64_t *out);
+/* graph g's cores from the last kc_batch_run (V_g entries) */
+KC_API kc_status kc_batch_graph_cores(const kc_batch *b, uint32_t g, uint32_t *out);
 KC_API kc_status kc_batch_dims(const kc_batch *b, uint32_t *G, uint64_t *vertices,
                                uint64_t *entries);
 
@@ -342,6 +352,94 @@ KC_API kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u, cons
                            uint64_t n, uint32_t *core_out, kc_update_stats *stats);
 /* current vertex count and adjacency entries (2x undirected edges) */
 KC_API kc_status kc_graph_dims(const kc_graph *g, uint32_t *V, uint64_t *nnz);
+
+/* ------------------------------------------------------------- serving */
+/* A request loop on one GPU (no counterpart in the reference).  Worker
+ * threads, each with its own HIP stream, admit queued requests as slots and
+ * HBM budget free up; small fresh-graph decompositions admitted together are
+ * fused into one batched launch; requests on a resident graph apply in
+ * submission order.  Results are bit-identical to running each request
+ * alone. */
+typedef struct kc_server kc_server;
+
+typedef struct kc_server_opts {
+    uint32_t slots;          /* worker threads / streams (default 4) */
+    uint32_t max_batch;      /* most requests fused into one launch (default 256) */
+    uint32_t fuse_max_scale; /* fresh R-MAT graphs up to this scale are fused (default 16) */
+    uint32_t reserved0;
+    uint64_t hbm_budget;     /* bytes of HBM the requests in flight may use (default: half) */
+    uint64_t reserved[4];
+} kc_server_opts;
+
+typedef enum kc_req_kind {
+    KC_REQ_DECOMPOSE_RMAT = 0,  /* build R-MAT(scale, edge_factor, seed) in HBM, decompose */
+    KC_REQ_UPDATE = 1,          /* kc_update on a resident graph */
+    KC_REQ_DECOMPOSE_GRAPH = 2  /* decompose a resident graph */
+} kc_req_kind;
+
+typedef struct kc_request {
+    uint32_t kind;  /* kc_req_kind */
+    uint32_t graph; /* resident graph id (UPDATE, DECOMPOSE_GRAPH) */
+    uint32_t op;    /* kc_update_op (UPDATE) */
+    uint32_t scale; /* R-MAT scale (DECOMPOSE_RMAT), a,b,c = .57,.19,.19 */
+    double edge_factor;
+    uint64_t seed;
+    uint64_t n;           /* UPDATE: pairs in u, v (copied at submit) */
+    const uint32_t *u, *v;
+    uint32_t *core_out;   /* optional host buffer for the V cores; caller keeps it
+                             valid until the completion is polled */
+    uint64_t tag;         /* caller's, returned in the completion */
+    uint64_t reserved[4];
+} kc_request;
+
+typedef struct kc_completion {
+    uint64_t id;
+    uint64_t tag;
+    int32_t status; /* kc_status */
+    uint32_t kind;
+    uint32_t V;
+    uint32_t kmax;  /* fresh decompositions */
+    uint32_t fused; /* requests in the launch that served this one */
+    uint32_t reserved0;
+    double submit_ms, start_ms, done_ms; /* server clock */
+    uint64_t checksum; /* kc_core_checksum of the resulting core vector */
+    uint64_t applied;  /* UPDATE: edges applied */
+    uint64_t reserved[4];
+} kc_completion;
+
+typedef struct kc_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t jobs;            /* units of work started (a fused launch counts once) */
+    uint64_t fused_jobs;      /* jobs serving more than one request */
+    uint64_t fused_requests;  /* requests served by those */
+    uint64_t admission_waits; /* admission stopped at a request that did not fit */
+    uint64_t max_busy;        /* most workers busy at once */
+    uint64_t max_inflight_bytes;
+    uint64_t hbm_budget;
+    uint64_t reserved[4];
+} kc_server_stats;
+
+KC_API kc_status kc_server_create(int device, const kc_server_opts *opts, kc_server **out);
+/* a resident graph (R-MAT, .57/.19/.19) that UPDATE requests maintain */
+KC_API kc_status kc_server_gen_rmat_graph(kc_server *s, uint32_t scale, double edge_factor,
+                                          uint64_t seed, uint32_t *graph_id);
+/* host copy of a resident graph once its queued requests are done */
+KC_API kc_status kc_server_graph_download(kc_server *s, uint32_t graph_id,
+                                          kc_host_graph **out);
+KC_API kc_status kc_server_submit(kc_server *s, const kc_request *r, uint64_t *id);
+/* up to max completions; waits up to wait_ms (-1: until one is available or
+ * nothing is outstanding, 0: no wait) */
+KC_API kc_status kc_server_poll(kc_server *s, kc_completion *out, uint32_t max, uint32_t *n,
+                                int wait_ms);
+KC_API kc_status kc_server_drain(kc_server *s);
+KC_API double kc_server_now_ms(const kc_server *s);
+KC_API kc_status kc_server_get_stats(kc_server *s, kc_server_stats *out);
+/* waits for outstanding requests, then frees everything */
+KC_API void kc_server_destroy(kc_server *s);
+
+/* order-independent checksum of a core vector, sum over v of
+ * splitmix64(v << 32 | core[v]) mod 2^64 (host computation) */
+KC_API uint64_t kc_core_checksum(const uint32_t *core, uint32_t V);
 
 /* ------------------------------------------------------------ autotuner */
 /* Time the launch candidates of `method` on graph g (block size, virtual

@@ -5,8 +5,8 @@ C++ drop-ins for the reference executables; this package is the Python view
 of that ABI.  See pico_amd.kcore.
 """
 from .kcore import (  # noqa: F401
-    METHODS, Context, DeviceBatch, DeviceGraph, HostBatch, HostGraph, KCoreError, lib, ref_peel,
-    vgraph_decompose,
+    METHODS, Context, DeviceBatch, DeviceGraph, HostBatch, HostGraph, KCoreError, Server,
+    core_checksum, lib, ref_peel, vgraph_decompose,
 )
 
 __version__ = "0.1.0"

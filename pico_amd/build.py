@@ -36,6 +36,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "kc_capi.cpp"),
     os.path.join(CSRC, "kc_batch_capi.cpp"),
     os.path.join(CSRC, "kc_dyn_capi.cpp"),
+    os.path.join(CSRC, "kc_server.cpp"),
     os.path.join(HOST, "kc_host.cpp"),
 ]
 

@@ -491,22 +491,25 @@ kc_status kc_graph_download(const kc_graph *g, kc_host_graph **out)
     return KC_OK;
 }
 
-kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min, uint32_t scale_max,
-                            double ef_min, double ef_max, double a, double b, double c,
-                            uint64_t seed, kc_batch **out)
+kc_status kc_batch_gen_rmat_list(kc_ctx *ctx, uint32_t G, const uint32_t *scale,
+                                 const uint64_t *m_gen, const uint64_t *seeds, double a, double b,
+                                 double c, kc_batch **out)
 {
     if (!ctx || !out || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
         return fail(KC_EINVAL, "bad R-MAT parameters");
-    if (scale_min == 0 || scale_max > 31 || scale_min > scale_max || ef_min < 0 ||
-        ef_max < ef_min || G == 0xffffffffu)
+    if (G == 0xffffffffu || (G && (!scale || !m_gen || !seeds)))
         return fail(KC_EINVAL, "bad batch parameters");
+    for (uint32_t g = 0; g < G; g++)
+        if (scale[g] == 0 || scale[g] > 31)
+            return fail(KC_EINVAL, "graph %u: R-MAT scale %u out of range", g, scale[g]);
     KC_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto *bt = new (std::nothrow) kc_batch;
     if (!bt)
         return fail(KC_ENOMEM, "host allocation");
     bt->ctx = ctx;
     bt->G = G;
-    kch::rmat_batch_draws(G, scale_min, scale_max, ef_min, ef_max, seed, bt->scale, bt->m_gen);
+    bt->scale.assign(scale, scale + G);
+    bt->m_gen.assign(m_gen, m_gen + G);
     bt->vtx_off.assign(G + 1ull, 0);
     for (uint32_t g = 0; g < G; g++)
         bt->vtx_off[g + 1] = bt->vtx_off[g] + (1ull << bt->scale[g]);
@@ -517,7 +520,7 @@ kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min, uint32_
     s.scale = bt->scale.data();
     s.m_gen = bt->m_gen.data();
     s.vtx_off = bt->vtx_off.data();
-    s.seed = seed;
+    s.seeds = seeds;
     s.a = a;
     s.ab = a + b;
     s.abc = a + b + c;
@@ -542,6 +545,57 @@ kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min, uint32_
         return st;
     }
     *out = bt;
+    return KC_OK;
+}
+
+kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min, uint32_t scale_max,
+                            double ef_min, double ef_max, double a, double b, double c,
+                            uint64_t seed, kc_batch **out)
+{
+    if (scale_min == 0 || scale_max > 31 || scale_min > scale_max || ef_min < 0 ||
+        ef_max < ef_min || G == 0xffffffffu)
+        return fail(KC_EINVAL, "bad batch parameters");
+    std::vector<uint32_t> sc;
+    std::vector<uint64_t> mg, seeds(G);
+    kch::rmat_batch_draws(G, scale_min, scale_max, ef_min, ef_max, seed, sc, mg);
+    for (uint32_t g = 0; g < G; g++)
+        seeds[g] = seed + g;
+    return kc_batch_gen_rmat_list(ctx, G, sc.data(), mg.data(), seeds.data(), a, b, c, out);
+}
+
+kc_status kc_batch_checksums(kc_batch *b, uint64_t *out)
+{
+    if (!b || (b->G && !out))
+        return fail(KC_EINVAL, "null argument");
+    if (!b->G)
+        return KC_OK;
+    hipStream_t st = b->ctx->stream;
+    uint64_t *d = nullptr;
+    KC_HIP(hipMallocAsync((void **)&d, (size_t)b->G * 8, st), "checksum buffer");
+    hipError_t e = hipSuccess;
+    for (uint32_t g = 0; g < b->G && e == hipSuccess; g++)
+        e = kc::core_checksum(b->d_core + b->vtx_off[g],
+                              (uint32_t)(b->vtx_off[g + 1] - b->vtx_off[g]), d + g, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, (size_t)b->G * 8, hipMemcpyDeviceToHost, st);
+    hipError_t e2 = hipFreeAsync(d, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e == hipSuccess)
+        e = e2;
+    return e == hipSuccess ? KC_OK : hip_fail(e, "batch checksums");
+}
+
+kc_status kc_batch_graph_cores(const kc_batch *b, uint32_t g, uint32_t *out)
+{
+    if (!b || !out)
+        return fail(KC_EINVAL, "null argument");
+    if (g >= b->G)
+        return fail(KC_EINVAL, "graph %u out of range (G=%u)", g, b->G);
+    uint64_t V = b->vtx_off[g + 1] - b->vtx_off[g];
+    if (V)
+        KC_HIP(hipMemcpy(out, b->d_core + b->vtx_off[g], V * 4, hipMemcpyDeviceToHost),
+               "core copy");
     return KC_OK;
 }
 

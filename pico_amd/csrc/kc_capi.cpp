@@ -435,6 +435,16 @@ const uint32_t *kc_graph_core_device(const kc_graph *g) { return g ? g->result :
 } // extern "C"
 
 // ----------------------------------------------------------- dispatching
+// Persistent engines wait at grid barriers for every workgroup of their
+// grid; two of them sharing a device from different streams could each hold
+// CUs the other's unscheduled workgroups need.  One lock per device keeps
+// them apart (everything else still overlaps).
+static std::mutex &coop_mutex(int device)
+{
+    static std::mutex mu[64];
+    return mu[(unsigned)device & 63];
+}
+
 __global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
 {
     uint32_t m = 0;
@@ -513,6 +523,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         p.trace = (uint64_t *)g->trace.p;
         p.trace_cap = KC_TRACE_CAP;
     }
+    std::unique_lock<std::mutex> coop(coop_mutex(c->device));
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_peel(p, c->stream);
     if (e != hipSuccess)
@@ -640,6 +651,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         p.trace = (uint64_t *)g->trace.p;
         p.trace_cap = KC_TRACE_CAP;
     }
+    std::unique_lock<std::mutex> coop(coop_mutex(c->device));
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_hindex(p, c->stream);
     if (e != hipSuccess)

@@ -112,7 +112,8 @@ struct GenSpec {
     const uint32_t *scale = nullptr;   // host, G (R-MAT)
     const uint64_t *m_gen = nullptr;   // host, G: generated pairs
     const uint64_t *vtx_off = nullptr; // host, G+1
-    uint64_t seed = 0;                 // graph g uses seed + g
+    uint64_t seed = 0;                 // graph g uses seed + g ...
+    const uint64_t *seeds = nullptr;   // ... or seeds[g] (host, G) when given
     double a = 0, ab = 0, abc = 0;     // R-MAT thresholds as the host computes them
     const uint32_t *thr = nullptr;     // device, Chung-Lu alias table (G == 1)
     const uint32_t *alias = nullptr;
@@ -157,6 +158,9 @@ hipError_t dyn_to_csr(const uint64_t *off, const uint32_t *len, const uint32_t *
                       const void *rowptr, bool wide, uint32_t *col, uint32_t V, hipStream_t st);
 hipError_t dyn_changed(const uint32_t *a, const uint32_t *b, uint32_t V, uint32_t *n,
                        hipStream_t st);
+
+// order-independent checksum of a core vector: sum of mix64(v << 32 | core[v])
+hipError_t core_checksum(const uint32_t *core, uint32_t V, uint64_t *out, hipStream_t st);
 
 hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st);
 int batch_occupancy(uint32_t block);

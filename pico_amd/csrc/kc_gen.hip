@@ -296,7 +296,7 @@ hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std
             uint32_t g = g0 + j;
             mpre[j] = npairs;
             npairs += s.m_gen[g];
-            gkey[j] = h_mix64((s.seed + g) ^ 0x524d41542d67656eull);
+            gkey[j] = h_mix64((s.seeds ? s.seeds[g] : s.seed + g) ^ 0x524d41542d67656eull);
             vrel[j] = vo[g] - v0;
             scale_h[j] = s.kind == GEN_RMAT ? s.scale[g] : 0;
         }
@@ -375,6 +375,28 @@ hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std
     }
     GEN_TRY(hipStreamSynchronize(st));
     return hipSuccess;
+}
+
+__global__ void checksum_kernel(const uint32_t *core, uint32_t V, unsigned long long *out)
+{
+    uint64_t h = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        h += d_mix64(((uint64_t)v << 32) | core[v]);
+    for (int o = 32; o >= 1; o >>= 1)
+        h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0 && h)
+        atomicAdd(out, (unsigned long long)h);
+}
+
+hipError_t core_checksum(const uint32_t *core, uint32_t V, uint64_t *out, hipStream_t st)
+{
+    hipError_t e = hipMemsetAsync(out, 0, 8, st);
+    if (e != hipSuccess || !V)
+        return e;
+    uint32_t blocks = (V + 255) / 256;
+    hipLaunchKernelGGL(checksum_kernel, dim3(blocks < 2048 ? blocks : 2048), dim3(256), 0, st,
+                       core, V, (unsigned long long *)out);
+    return hipGetLastError();
 }
 
 } // namespace kc

@@ -301,6 +301,14 @@ int gen_rmat_batch(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef
     return 0;
 }
 
+uint64_t core_checksum(const uint32_t *core, uint32_t V)
+{
+    uint64_t h = 0;
+    for (uint32_t v = 0; v < V; v++)
+        h += mix64(((uint64_t)v << 32) | core[v]);
+    return h;
+}
+
 // ------------------------------------------------------------- readers -----
 static bool slurp(const char *path, std::vector<char> &buf, std::string &err)
 {

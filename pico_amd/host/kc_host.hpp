@@ -100,6 +100,9 @@ int gen_rmat_batch(uint32_t G, uint32_t scale_min, uint32_t scale_max, double ef
 // each graph's arrays are freed once copied
 int batch_from_graphs(std::vector<kc_host_graph> &gs, kc_host_batch &out, bool release);
 
+// the device checksum of a core vector, computed on the host
+uint64_t core_checksum(const uint32_t *core, uint32_t V);
+
 int read_coo(const char *path, kc_host_graph &g, std::string &err);
 int read_dedup(const char *path, kc_host_graph &g, std::string &err);
 int write_edgelist(const kc_host_graph &g, const char *path, std::string &err);

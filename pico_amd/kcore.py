@@ -114,6 +114,46 @@ class UpdateStats(ctypes.Structure):
                 if not k.startswith("reserved")}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("fuse_max_scale", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("hbm_budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("graph", ctypes.c_uint32),
+                ("op", ctypes.c_uint32), ("scale", ctypes.c_uint32),
+                ("edge_factor", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("n", ctypes.c_uint64), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("core_out", ctypes.c_void_p), ("tag", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64),
+                ("status", ctypes.c_int32), ("kind", ctypes.c_uint32),
+                ("V", ctypes.c_uint32), ("kmax", ctypes.c_uint32),
+                ("fused", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("checksum", ctypes.c_uint64),
+                ("applied", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if not k.startswith("reserved")}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "submitted", "completed", "failed", "jobs", "fused_jobs", "fused_requests",
+        "admission_waits", "max_busy", "max_inflight_bytes", "hbm_budget")] + \
+        [("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_
+                if not k.startswith("reserved")}
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -196,6 +236,18 @@ def lib():
         "kc_update": ([_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp,
                        ctypes.POINTER(UpdateStats)], S),
         "kc_graph_dims": ([_vp, _u32p, _u64p], S),
+        "kc_server_create": ([ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(_vp)], S),
+        "kc_server_gen_rmat_graph": ([_vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
+                                      _u32p], S),
+        "kc_server_graph_download": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp)], S),
+        "kc_server_submit": ([_vp, ctypes.POINTER(Request), _u64p], S),
+        "kc_server_poll": ([_vp, ctypes.POINTER(Completion), ctypes.c_uint32, _u32p,
+                            ctypes.c_int], S),
+        "kc_server_drain": ([_vp], S),
+        "kc_server_now_ms": ([_vp], ctypes.c_double),
+        "kc_server_get_stats": ([_vp, ctypes.POINTER(ServerStats)], S),
+        "kc_server_destroy": ([_vp], None),
+        "kc_core_checksum": ([_vp, ctypes.c_uint32], ctypes.c_uint64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -574,6 +626,103 @@ class DeviceBatch:
     def close(self):
         if self._h:
             lib().kc_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def core_checksum(core):
+    """kc_core_checksum: sum over v of splitmix64(v << 32 | core[v]) mod 2^64."""
+    c = np.ascontiguousarray(core, np.uint32)
+    return int(lib().kc_core_checksum(_ptr(c), c.size))
+
+
+class Server:
+    """kc_server: the request loop (admission by HBM budget, fused small
+    decompositions, resident graphs updated in submission order)."""
+
+    KIND = {"rmat": 0, "update": 1, "decompose": 2}
+
+    def __init__(self, device=0, slots=4, max_batch=256, fuse_max_scale=16, hbm_budget=0):
+        o = ServerOpts()
+        o.slots, o.max_batch, o.fuse_max_scale, o.hbm_budget = (slots, max_batch,
+                                                                fuse_max_scale, hbm_budget)
+        self._h = _vp()
+        _check(lib().kc_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
+        self._keep = {}
+
+    def gen_rmat_graph(self, scale, edge_factor=16.0, seed=1):
+        gid = ctypes.c_uint32(0)
+        _check(lib().kc_server_gen_rmat_graph(self._h, scale, edge_factor, seed,
+                                              ctypes.byref(gid)))
+        return gid.value
+
+    def download_graph(self, gid):
+        h = _vp()
+        _check(lib().kc_server_graph_download(self._h, gid, ctypes.byref(h)))
+        return HostGraph(h)
+
+    def _submit(self, r, keep):
+        rid = ctypes.c_uint64(0)
+        _check(lib().kc_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        if keep is not None:
+            self._keep[rid.value] = keep
+        return rid.value
+
+    def submit_rmat(self, scale, edge_factor=16.0, seed=1, tag=0, core_out=None):
+        r = Request()
+        r.kind, r.scale, r.edge_factor, r.seed, r.tag = 0, scale, edge_factor, seed, tag
+        if core_out is not None:
+            r.core_out = core_out.ctypes.data
+        return self._submit(r, core_out)
+
+    def submit_update(self, gid, op, u, v, tag=0, core_out=None):
+        uu = np.ascontiguousarray(u, np.uint32)
+        vv = np.ascontiguousarray(v, np.uint32)
+        r = Request()
+        r.kind, r.graph, r.op, r.tag = 1, gid, {"insert": 0, "delete": 1}[op], tag
+        r.n, r.u, r.v = uu.size, uu.ctypes.data if uu.size else None, \
+            vv.ctypes.data if vv.size else None
+        if core_out is not None:
+            r.core_out = core_out.ctypes.data
+        return self._submit(r, core_out)
+
+    def submit_decompose(self, gid, tag=0, core_out=None):
+        r = Request()
+        r.kind, r.graph, r.tag = 2, gid, tag
+        if core_out is not None:
+            r.core_out = core_out.ctypes.data
+        return self._submit(r, core_out)
+
+    def poll(self, max_n=256, wait_ms=-1):
+        buf = (Completion * max_n)()
+        n = ctypes.c_uint32(0)
+        _check(lib().kc_server_poll(self._h, buf, max_n, ctypes.byref(n), wait_ms))
+        out = []
+        for i in range(n.value):
+            d = buf[i].as_dict()
+            self._keep.pop(d["id"], None)
+            out.append(d)
+        return out
+
+    def drain(self):
+        _check(lib().kc_server_drain(self._h))
+
+    def now_ms(self):
+        return float(lib().kc_server_now_ms(self._h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().kc_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().kc_server_destroy(self._h)
             self._h = None
 
     def __del__(self):
