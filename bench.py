@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per s")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--only", choices=["peel", "batch", "serve"], default=None)
     return ap.parse_args()
 
 
@@ -84,6 +89,114 @@ def traffic_from_profile(kernel_key):
             return json.load(f).get(kernel_key, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def _pct(xs, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(xs), q)), 3) if len(xs) else None
+
+
+def bench_serving(args):
+    """Poisson request stream against kc_server: fresh R-MAT decompositions
+    (scale uniform over 10..20, edge factor 16) and 10,000-edge update batches
+    (uniform random insertions / deletions of present edges) on a resident
+    R-MAT s20 ef16 graph.  Latency = completion - arrival (submission happens
+    at the arrival instant).  Every result is then re-served alone and
+    compared; the resident graph's final cores are checked against the
+    oracle."""
+    import threading
+
+    import numpy as np
+
+    import oracle
+    import pico_amd as pa
+
+    rng = np.random.default_rng(args.seed + 77)
+    n = args.serve_requests
+    base_scale, base_ef, base_seed = 20, 16.0, args.seed + 5
+    # the workload, fixed up front: kinds, arrival times, parameters
+    srv = pa.Server(slots=args.serve_slots, fuse_max_scale=16)
+    gid = srv.gen_rmat_graph(base_scale, base_ef, seed=base_seed)
+    base = srv.download_graph(gid)
+    rp, col = base.rowptr, base.col
+    src = np.repeat(np.arange(base.V, dtype=np.uint32), np.diff(rp).astype(np.int64))
+    up = np.flatnonzero(src < col)
+    del_pool = rng.permutation(up)
+    reqs, dp = [], 0
+    for i in range(n):
+        if rng.random() < 0.5:
+            reqs.append(("rmat", int(rng.integers(10, 21)), 16.0, 10_000 + i))
+        elif rng.random() < 0.5:
+            u = rng.integers(0, base.V, 10_000, dtype=np.uint32)
+            v = rng.integers(0, base.V, 10_000, dtype=np.uint32)
+            reqs.append(("update", "insert", u, v))
+        else:
+            pick = del_pool[dp: dp + 10_000]
+            dp += 10_000
+            reqs.append(("update", "delete", src[pick].copy(), col[pick].copy()))
+    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    arrive = np.cumsum(gaps) - gaps[0]
+    done = {}
+
+    def collect(total):
+        while len(done) < total:
+            for c in srv.poll(wait_ms=200):
+                done[c["tag"]] = c
+
+    th = threading.Thread(target=collect, args=(n,))
+    th.start()
+    t0 = time.perf_counter()
+    for k, r in enumerate(reqs):
+        dt = arrive[k] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        if r[0] == "rmat":
+            srv.submit_rmat(r[1], r[2], seed=r[3], tag=k)
+        else:
+            srv.submit_update(gid, r[1], r[2], r[3], tag=k)
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    final = srv.download_graph(gid)
+    srv.close()
+    lat = {k: c["done_ms"] - c["submit_ms"] for k, c in done.items()}
+    dec = [lat[k] for k, r in enumerate(reqs) if r[0] == "rmat"]
+    upd = [lat[k] for k, r in enumerate(reqs) if r[0] != "rmat"]
+    first = min(c["submit_ms"] for c in done.values())
+    last = max(c["done_ms"] for c in done.values())
+    # every result again, alone (one slot, one request at a time)
+    alone = pa.Server(slots=1, max_batch=1)
+    gid2 = alone.gen_rmat_graph(base_scale, base_ef, seed=base_seed)
+    same = 0
+    for k, r in enumerate(reqs):
+        if r[0] == "rmat":
+            alone.submit_rmat(r[1], r[2], seed=r[3], tag=k)
+        else:
+            alone.submit_update(gid2, r[1], r[2], r[3], tag=k)
+        c = alone.poll(wait_ms=-1)[0]
+        same += int(c["checksum"] == done[k]["checksum"] and c["status"] == 0
+                    and done[k]["status"] == 0)
+    alone.close()
+    last_upd = max((k for k, r in enumerate(reqs) if r[0] != "rmat"), default=None)
+    final_ok = None
+    if last_upd is not None:
+        want = oracle.bz(final.rowptr, final.col)
+        final_ok = bool(done[last_upd]["checksum"] == pa.core_checksum(want))
+    return {
+        "workload": "Poisson arrivals at %.1f/s: 50%% fresh R-MAT decompositions (scale uniform "
+                    "10..20, edge factor 16), 50%% 10,000-edge insert/delete batches on a "
+                    "resident R-MAT s20 ef16 graph" % args.serve_rate,
+        "requests": n, "rate_per_s": args.serve_rate, "slots": args.serve_slots,
+        "requests_per_s": round(n / ((last - first) * 1e-3), 2),
+        "p50_ms": _pct(list(lat.values()), 50), "p99_ms": _pct(list(lat.values()), 99),
+        "decompose_p50_ms": _pct(dec, 50), "decompose_p99_ms": _pct(dec, 99),
+        "update_p50_ms": _pct(upd, 50), "update_p99_ms": _pct(upd, 99),
+        "max_ms": round(max(lat.values()), 3), "wall_s": round(wall, 2),
+        "fused_jobs": st["fused_jobs"], "fused_requests": st["fused_requests"],
+        "max_busy_slots": st["max_busy"],
+        "max_inflight_gb": round(st["max_inflight_bytes"] / 1e9, 3),
+        "same_alone": "%d/%d" % (same, n), "final_cores_vs_oracle": final_ok,
+    }
 
 
 def bench_batched(ctx, args):
@@ -158,6 +271,14 @@ def main():
 
     import pico_amd as pa
 
+    if args.only in ("batch", "serve"):  # one section alone (profiling runs)
+        if args.only == "serve":
+            print(json.dumps({"serving": bench_serving(args)}), flush=True)
+        else:
+            c = pa.Context(0)
+            print(json.dumps({"batched": bench_batched(c, args)}), flush=True)
+            c.close()
+        return
     ctx = pa.Context(0)
     t0 = time.time()
     # built in HBM (the host generator's graph, edge for edge); resident
@@ -215,6 +336,7 @@ def main():
         parity = bool(np.array_equal(core, ref) and np.array_equal(hcore, ref))
 
     batched = None if args.no_batch else bench_batched(ctx, args)
+    serving = None if args.no_serve else bench_serving(args)
 
     out = {
         "metric": "k-core GTEPS + % HBM roofline (R-MAT s22); batched vs plain graphs/s; p99",
@@ -252,6 +374,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     dg.close()
