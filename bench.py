@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only", choices=["peel", "batch", "serve"], default=None)
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="write profiles/pmc_traffic.json from two rocprofv3 --pmc "
+                         "databases (FETCH_SIZE, WRITE_SIZE) of this script's runs")
     return ap.parse_args()
 
 
@@ -260,6 +263,55 @@ def bench_batched(ctx, args):
     return out
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def _pmc_per_kernel(db_path, counter):
+    """Mean value of `counter` per dispatch, by kernel name, from a rocprofv3
+    rocpd database."""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    view, name_col, disp_col, cnt_col, val_col = ("pmc_events", "name", "dispatch_id",
+                                                   "counter_name", "counter_value")
+    rows = cur.execute("select %s, %s, sum(%s) from %s where %s = ? group by %s, %s"
+                       % (name_col, disp_col, val_col, view, cnt_col, name_col, disp_col),
+                       (counter,)).fetchall()
+    out = {}
+    for name, _, v in rows:
+        out.setdefault(name, []).append(float(v))
+    return {k: sum(v) / len(v) for k, v in out.items()}
+
+
+def summarize_pmc(paths):
+    """profiles/pmc_traffic.json: HBM bytes per launch of the dominant
+    kernels = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled so that the
+    copy probe's counters read the bytes it is known to move."""
+    fetch = _pmc_per_kernel(paths[0], "FETCH_SIZE")
+    write = _pmc_per_kernel(paths[1], "WRITE_SIZE")
+
+    def pick(d, key):
+        ks = [k for k in d if key in k]
+        return d[ks[0]] if ks else None
+
+    cf, cw = pick(fetch, "copy_probe_kernel"), pick(write, "copy_probe_kernel")
+    sf = COPY_PROBE_BYTES / (cf * 1024.0)
+    sw = COPY_PROBE_BYTES / (cw * 1024.0)
+    out = {"copy_probe": {"bytes": COPY_PROBE_BYTES, "fetch_kib": cf, "write_kib": cw,
+                          "fetch_scale": sf, "write_scale": sw},
+           "sources": [os.path.basename(p) for p in paths]}
+    for key, kname in (("peel", "peel_kernel"), ("hindex", "hindex_kernel"),
+                       ("batch", "batch_kernel")):
+        f, w = pick(fetch, kname), pick(write, kname)
+        if f is None or w is None:
+            continue
+        out[key] = {"kernel": kname, "fetch_kib": f, "write_kib": w,
+                    "bytes_per_launch": int(f * 1024.0 * sf + w * 1024.0 * sw)}
+    with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -271,6 +323,11 @@ def main():
 
     import pico_amd as pa
 
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc)
+        return
+    if args.only == "peel":
+        args.no_batch = args.no_serve = True
     if args.only in ("batch", "serve"):  # one section alone (profiling runs)
         if args.only == "serve":
             print(json.dumps({"serving": bench_serving(args)}), flush=True)
@@ -288,6 +345,11 @@ def main():
     g = dg.download()  # host copy for the oracle and the CPU baseline
     V, nnz = g.V, g.nnz
     m = nnz // 2
+
+    # streaming-copy probe: achievable HBM bandwidth on this device, and the
+    # calibration kernel for the HBM byte counters
+    copy_ms = ctx.probe_copy(COPY_PROBE_BYTES, 5)
+    copy_gbs = 2 * COPY_PROBE_BYTES / (copy_ms * 1e-3) / 1e9
 
     # ---------------------------------------------------- plain peel steps
     for _ in range(args.warmup):
@@ -361,7 +423,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic_from_profile("peel"),
                      "kernel": "peel_kernel", "kernel_ms": round(kernel_ms, 4),
-                     "algorithmic_bytes": b_alg},
+                     "algorithmic_bytes": b_alg,
+                     "copy_probe_gbs": round(copy_gbs, 1)},
         "hindex": {"method": args.hindex_method, "kernel_ms": round(h_kernel_ms, 4),
                    "gteps": round(m / (h_kernel_ms * 1e-3) / 1e9, 4),
                    "rounds": hst["iterations"], "changed": hst["changed"],

@@ -441,6 +441,12 @@ KC_API void kc_server_destroy(kc_server *s);
  * splitmix64(v << 32 | core[v]) mod 2^64 (host computation) */
 KC_API uint64_t kc_core_checksum(const uint32_t *core, uint32_t V);
 
+/* ---------------------------------------------------------- calibration */
+/* Streaming device copy of `bytes` (two buffers), `reps` timed launches;
+ * *ms = mean time of one.  Reads and writes `bytes` each: the reference
+ * point for achievable HBM bandwidth and for scaling HBM byte counters. */
+KC_API kc_status kc_probe_copy(kc_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms);
+
 /* ------------------------------------------------------------ autotuner */
 /* Time the launch candidates of `method` on graph g (block size, virtual
  * node split width, local frontier capacity for the peels; block size for

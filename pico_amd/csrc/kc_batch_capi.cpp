@@ -820,6 +820,46 @@ void kc_batch_free(kc_batch *b)
     delete b;
 }
 
+// ------------------------------------------------------------ calibration
+kc_status kc_probe_copy(kc_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms)
+{
+    if (!ctx || !ms || bytes < 16)
+        return fail(KC_EINVAL, "bad argument");
+    bytes &= ~15ull;
+    KC_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    void *a = nullptr, *b = nullptr;
+    KC_HIP(hipMalloc(&a, bytes), "probe buffer");
+    hipError_t e = hipMalloc(&b, bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess)
+        e = hipMemsetAsync(a, 1, bytes, ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+        e = kc::copy_probe(a, b, bytes, ctx->stream); // warm-up
+    if (e == hipSuccess)
+        e = hipEventRecord(e0, ctx->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; r++)
+        e = kc::copy_probe(a, b, bytes, ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess)
+        return hip_fail(e, "copy probe");
+    *ms = reps ? t / reps : 0.0;
+    return KC_OK;
+}
+
 // ---------------------------------------------------------------- autotuner
 kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *best,
                       double *best_ms)

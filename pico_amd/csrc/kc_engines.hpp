@@ -159,6 +159,8 @@ hipError_t dyn_to_csr(const uint64_t *off, const uint32_t *len, const uint32_t *
 hipError_t dyn_changed(const uint32_t *a, const uint32_t *b, uint32_t V, uint32_t *n,
                        hipStream_t st);
 
+// streaming copy of bytes (multiple of 16) for bandwidth calibration
+hipError_t copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 // order-independent checksum of a core vector: sum of mix64(v << 32 | core[v])
 hipError_t core_checksum(const uint32_t *core, uint32_t V, uint64_t *out, hipStream_t st);
 

@@ -377,6 +377,26 @@ hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std
     return hipSuccess;
 }
 
+// plain streaming copy (16 B per lane per step): the bandwidth probe the
+// HBM counters of the engines are scaled against
+__global__ void copy_probe_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                  uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+hipError_t copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t st)
+{
+    uint64_t n = bytes / 16;
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(8192), dim3(256), 0, st, (const uint4 *)src,
+                       (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
 __global__ void checksum_kernel(const uint32_t *core, uint32_t V, unsigned long long *out)
 {
     uint64_t h = 0;

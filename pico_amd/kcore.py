@@ -248,6 +248,8 @@ def lib():
         "kc_server_get_stats": ([_vp, ctypes.POINTER(ServerStats)], S),
         "kc_server_destroy": ([_vp], None),
         "kc_core_checksum": ([_vp, ctypes.c_uint32], ctypes.c_uint64),
+        "kc_probe_copy": ([_vp, ctypes.c_uint64, ctypes.c_uint32,
+                           ctypes.POINTER(ctypes.c_double)], S),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -464,6 +466,12 @@ class Context:
         h = _vp()
         _check(lib().kc_graph_upload_hg(self._h, graph._h, rowptr_bytes, ctypes.byref(h)))
         return DeviceGraph(self, h, graph.V)
+
+    def probe_copy(self, nbytes=1 << 30, reps=5):
+        """Mean ms of a streaming copy of nbytes (reads and writes nbytes)."""
+        ms = ctypes.c_double(0)
+        _check(lib().kc_probe_copy(self._h, nbytes, reps, ctypes.byref(ms)))
+        return ms.value
 
     def gen_rmat(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, rowptr_bytes=4):
         """HostGraph.rmat's graph, built in HBM."""
