@@ -299,7 +299,7 @@ def summarize_pmc(paths):
     sw = COPY_PROBE_BYTES / (cw * 1024.0)
     out = {"copy_probe": {"bytes": COPY_PROBE_BYTES, "fetch_kib": cf, "write_kib": cw,
                           "fetch_scale": sf, "write_scale": sw},
-           "sources": [os.path.basename(p) for p in paths]}
+           "sources": [os.path.relpath(p, ROOT) for p in paths]}
     for key, kname in (("peel", "peel_kernel"), ("hindex", "hindex_kernel"),
                        ("batch", "batch_kernel")):
         f, w = pick(fetch, kname), pick(write, kname)
