@@ -857,8 +857,8 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     if (e != hipSuccess)
         return e;
     void *args[] = {&a};
-    e = hipLaunchCooperativeKernel((const void *)hindex_kernel<BLOCK, R>, dim3(p.grid),
-                                   dim3(BLOCK), args, 0, st);
+    e = launch_persistent((const void *)hindex_kernel<BLOCK, R>, dim3(p.grid), dim3(BLOCK),
+                          args, st);
     return e;
 }
 

@@ -826,8 +826,8 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     if (e != hipSuccess)
         return e;
     void *args[] = {&a};
-    e = hipLaunchCooperativeKernel((const void *)peel_kernel<BLOCK, LQ, R>, dim3(p.grid),
-                                   dim3(BLOCK), args, 0, st);
+    e = launch_persistent((const void *)peel_kernel<BLOCK, LQ, R>, dim3(p.grid), dim3(BLOCK),
+                          args, st);
     if (e != hipSuccess)
         return e;
     if (p.mode == MODE_GPP && !p.gpp_compat && p.V)
