@@ -217,20 +217,18 @@ __device__ __forceinline__ uint32_t find_item(const uint32_t *pre, uint32_t n, u
     return lo;
 }
 
-// Launch a persistent grid-barrier kernel.  The cooperative launch checks
-// that the grid fits on the device at once.  When the runtime refuses it
-// because another process holds the device's cooperative-launch resources
-// (hipErrorIllegalState), the same grid -- sized to the occupancy limit -- is
-// launched normally: the barrier is the kernel's own (atomics, not the
-// runtime's), and its wall-clock limit turns a grid that cannot all be
-// resident into a reported time-out instead of a hang.
+// Launch a persistent grid-barrier kernel.  The grid is sized by the caller
+// to at most the occupancy limit (and at most two workgroups per CU), and
+// the barrier is the kernel's own (atomics, not the runtime's cooperative
+// groups), so a plain launch suffices; the runtime's cooperative launch is
+// not used because it is refused (after a long wait) while another process
+// holds the device's cooperative-launch resources.  Should the grid ever not
+// be resident at once, the barrier's wall-clock limit reports a time-out
+// instead of hanging.  Persistent launches of one process are serialized by
+// the caller.
 inline hipError_t launch_persistent(const void *fn, dim3 grid, dim3 block, void **args,
                                     hipStream_t st)
 {
-    hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, st);
-    if (e != hipErrorIllegalState)
-        return e;
-    (void)hipGetLastError();
     return hipLaunchKernel(fn, grid, block, args, 0, st);
 }
 
