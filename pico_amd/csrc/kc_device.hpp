@@ -82,6 +82,17 @@ __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Wait until this thread's outstanding global loads, stores and atomics have
+// completed (s_waitcnt vmcnt(0); expcnt and lgkmcnt left alone).  Orders
+// device-coherent accesses -- agent-scope atomics and atomic loads/stores --
+// against later ones without the L2 write-back and invalidation an
+// agent-scope fence costs on gfx950.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ void st_relaxed64(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
@@ -96,9 +107,10 @@ struct PhaseSlot {
     uint64_t sum2;
     uint32_t min0;  // engine-defined minimum (cleared to UINT32_MAX)
     uint32_t qtail; // items pushed to the shared list of this phase
-    uint32_t qhead; // grab counter for reading that list next phase
+    uint32_t qhead; // grab counter for reading that list
     uint32_t flag;  // engine-defined OR
-    uint32_t pad[2];
+    uint32_t busy;  // workgroups still in their own work this phase (starts at nblocks)
+    uint32_t qdone; // shared-list items fully processed this phase
 };
 
 struct GridSync {
@@ -111,8 +123,10 @@ struct GridSync {
     PhaseSlot slot[2];
 };
 
-__device__ __forceinline__ void slot_clear(PhaseSlot *s)
+__device__ __forceinline__ void slot_clear(PhaseSlot *s, uint32_t nblocks)
 {
+    st_relaxed(&s->busy, nblocks);
+    st_relaxed(&s->qdone, 0u);
     __hip_atomic_store(&s->sum0, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s->sum1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s->sum2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -135,19 +149,28 @@ __device__ __forceinline__ void sync_fail(GridSync *gs, uint32_t code)
 // Grid-wide barrier.  `gen` is this WG's count of barriers passed so far
 // (identical in every WG).  Returns false if the kernel must abort (timeout
 // or error raised by some WG); all WGs then leave through the same path.
+// kFlush = true publishes plain (non-atomic) stores across the grid with
+// agent-scope fences, which on gfx950 write back and invalidate the L2 of
+// every XCD.  An engine whose cross-workgroup data is only ever touched by
+// atomics and device-coherent (atomic) loads/stores passes false: the barrier
+// then only waits for each workgroup's outstanding memory operations, and the
+// read-only graph stays cached in L2 from one phase to the next.
+template <bool kFlush = true>
 __device__ __forceinline__ bool grid_barrier(GridSync *gs, uint32_t &gen)
 {
     __shared__ uint32_t s_ok;
+    vm_drain();
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t ok = 1;
-        __threadfence();
+        if (kFlush)
+            __threadfence();
         uint32_t nb = gs->nblocks;
         uint32_t arrived = atomicAdd(&gs->arrive, 1u);
         if (arrived == nb - 1) {
             st_relaxed(&gs->arrive, 0u);
-            slot_clear(&gs->slot[(gen + 1) & 1]);
-            __threadfence();
+            slot_clear(&gs->slot[(gen + 1) & 1], nb);
+            vm_drain();
             atomicAdd(&gs->gen, 1u);
         } else {
             uint64_t start = now_ticks();
@@ -164,7 +187,10 @@ __device__ __forceinline__ bool grid_barrier(GridSync *gs, uint32_t &gen)
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
+        if (kFlush)
+            __threadfence();
+        else
+            vm_drain();
         if (ld_relaxed(&gs->err))
             ok = 0;
         s_ok = ok;

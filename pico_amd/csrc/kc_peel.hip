@@ -51,6 +51,7 @@ constexpr uint32_t KC_AGG_LOG2 = 11; // decrement-aggregation slots per WG
 constexpr uint32_t KC_AGG = 1u << KC_AGG_LOG2;
 constexpr uint32_t KC_AGG_PROBES = 8;
 constexpr uint32_t KC_AGG_MARGIN = 32; // aggregate only targets this far above the level
+constexpr uint64_t KC_EMPTY = ~0ull;    // free shared-list slot (bucket / PeelOne)
 
 // LDS of one persistent workgroup.
 template <int BLOCK, int LQ> struct PeelLds {
@@ -66,6 +67,11 @@ template <int BLOCK, int LQ> struct PeelLds {
     uint32_t ncw;             // chunks dealt to this WG
     uint32_t use_list;        // ncw fits in chunks[]
     uint32_t grab;
+    uint32_t hn;     // slots reserved by the last grab of help_shared
+    uint32_t claims_tmp;
+    uint32_t hstate;
+    uint32_t tail_seen;
+    uint32_t hlen[BLOCK];
     uint32_t claims; // vertices claimed by this WG in the current phase
     uint32_t minv;   // min live degree seen by this WG's scan
     uint32_t live;   // any live vertex seen by the scan
@@ -111,7 +117,7 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
     if (want) {
         uint32_t p = base + lane_rank(m);
         if (p < a.list_cap)
-            list[p] = item;
+            st_relaxed64(&list[p], item);
         else
             sync_fail(a.gs, 2);
     }
@@ -169,7 +175,7 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
         uint32_t base = atomicAdd(&slot->qtail, nch);
         if (base + nch <= a.list_cap) {
             for (uint32_t c = 0; c < nch; c++)
-                list[base + c] = (uint64_t)u | ((uint64_t)c << 32);
+                st_relaxed64(&list[base + c], (uint64_t)u | ((uint64_t)c << 32));
         } else {
             sync_fail(a.gs, 2);
         }
@@ -272,7 +278,10 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         for (int k = 0; k < WALK_U; k++) {
             pre[k] = 0;
             if (val[k])
-                pre[k] = a.mode == MODE_GPP ? a.done[u[k] >> 6] : (uint64_t)a.deg[u[k]];
+                // GPP needs the exact claim word; the clamp pre-check may
+                // read a stale degree (stale values only ever lie above)
+                pre[k] = a.mode == MODE_GPP ? ld_relaxed64(&a.done[u[k] >> 6])
+                                            : (uint64_t)a.deg[u[k]];
         }
         uint32_t old[WALK_U];
         bool hit[WALK_U];
@@ -463,7 +472,8 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         for (uint32_t k = 0; k < U; k++) {
             uint32_t v = cc[k] * 64 + lane;
             lv[k] = dw[k] != ~0ull && !((dw[k] >> lane) & 1ull) && v < a.V;
-            d[k] = lv[k] ? a.deg[v] : 0xffffffffu;
+            // exact value: the cascades of earlier phases changed it by atomics
+            d[k] = lv[k] ? ld_relaxed(&a.deg[v]) : 0xffffffffu;
         }
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) {
@@ -537,7 +547,7 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
         uint32_t m = n_items - s < g ? n_items - s : g;
         uint32_t len = 0;
         if (threadIdx.x < m) {
-            uint64_t it = list[s + threadIdx.x];
+            uint64_t it = ld_relaxed64(&list[s + threadIdx.x]);
             uint32_t v = (uint32_t)it;
             R rs = a.rowptr[v];
             uint32_t d = (uint32_t)(a.rowptr[v + 1] - rs);
@@ -564,6 +574,156 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
         walk_items<BLOCK, LQ, R>(L, a, level, phase, m, 0);
         if (a.mode != MODE_GPP)
             drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+    }
+}
+
+// Bucket / PeelOne: once a workgroup has drained its own frontier it takes
+// items from this phase's shared list -- spilled claims and hub chunks other
+// workgroups pushed -- while they are still being produced, so a cascade
+// that one workgroup cannot hold is spread over all of them within the
+// level.  The list is a ticket queue: pushers reserve slots at the tail and
+// then write them; takers reserve slots at the head with an atomicAdd that
+// may run past the tail (those slots are filled by later pushes, or never).
+// A reserved slot is taken once it leaves KC_EMPTY and is emptied again.
+// Termination: `busy` counts workgroups still in their own work, `qdone` the
+// items whose processing (and the pushes it caused) is complete; once busy
+// is 0 and qdone, read before the tail, equals the tail, nothing can be
+// pushed any more and every slot at or past the tail is dropped.  Ready items are always processed
+// before waiting on the rest of a reservation, so no workgroup sits on work
+// another one waits for.
+template <int BLOCK, int LQ, typename R>
+__device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                            uint32_t phase, uint64_t &edges)
+{
+    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    uint64_t *list = (phase & 1) ? a.list1 : a.list0;
+    const uint32_t nb = a.gs->nblocks;
+    // Every access the protocol relies on is device-coherent (atomics, and
+    // atomic loads / stores of list slots), so ordering needs only that a
+    // WG's earlier accesses have completed before thread 0 moves a counter.
+    vm_drain();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicSub(&slot->busy, 1u); // this WG's own work is over
+    for (;;) {
+        // ---- reserve a range of slots
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = ld_relaxed(&slot->qtail), h = ld_relaxed(&slot->qhead);
+            uint32_t n = t > h ? (t - h + nb - 1) / nb : 1u;
+            n = n > (uint32_t)BLOCK ? (uint32_t)BLOCK : n;
+            L.grab = atomicAdd(&slot->qhead, n);
+            L.hn = n;
+        }
+        __syncthreads();
+        const uint32_t s = L.grab, n = L.hn;
+        const uint32_t p = s + threadIdx.x;
+        // slots past the list's capacity can never be written (a push there
+        // reports the overflow itself)
+        bool pending = threadIdx.x < n && p < a.list_cap;
+        uint64_t idle0 = 0;   // thread 0: start of the current wait
+        uint32_t backoff = 1; // idle polls sleep longer each time (up to ~4 us)
+        for (;;) {
+            // ---- take what is ready
+            uint64_t it = KC_EMPTY;
+            if (pending) {
+                it = ld_relaxed64(&list[p]);
+                if (it != KC_EMPTY) {
+                    st_relaxed64(&list[p], KC_EMPTY);
+                    pending = false;
+                }
+            }
+            if (threadIdx.x == 0)
+                L.claims_tmp = 0;
+            __syncthreads();
+            uint32_t len = 0;
+            bool have = it != KC_EMPTY;
+            uint32_t idx = 0;
+            {
+                uint64_t m = ballot(have);
+                uint32_t base = 0;
+                uint32_t lane = lane_id();
+                if (m && lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                    base = atomicAdd(&L.claims_tmp, (uint32_t)__popcll(m));
+                base = __shfl(base, __ffsll((unsigned long long)(m ? m : 1)) - 1, KC_WAVE);
+                idx = base + lane_rank(m);
+            }
+            __syncthreads();
+            const uint32_t k = L.claims_tmp; // items taken this round
+            if (have) {
+                uint32_t v = (uint32_t)it;
+                R rs = a.rowptr[v];
+                uint32_t d = (uint32_t)(a.rowptr[v + 1] - rs);
+                uint64_t off = (uint64_t)(uint32_t)(it >> 32) * a.hub_split;
+                uint32_t rem = (uint32_t)(d - off);
+                L.start[idx] = (uint64_t)rs + off;
+                L.hlen[idx] = rem < a.hub_split ? rem : a.hub_split;
+            }
+            __syncthreads();
+            if (k) {
+                len = threadIdx.x < k ? L.hlen[threadIdx.x] : 0u;
+                edges += len;
+                stage_prefix<BLOCK, LQ>(L, k, len);
+                walk_items<BLOCK, LQ, R>(L, a, level, phase, k, 0);
+                drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+                vm_drain(); // the pushes these items caused come first
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    atomicAdd(&slot->qdone, k);
+            }
+            // ---- anything left in the reservation?
+            if (threadIdx.x == 0) {
+                uint32_t st = 0; // 0 go on, 1 phase over, 2 error
+                if (ld_relaxed(&slot->busy) == 0) {
+                    // done first, tail second: if every item pushed by the
+                    // time `done` was read had been processed, and the tail
+                    // has not moved since, nothing is in flight
+                    vm_drain();
+                    uint32_t dn = ld_relaxed(&slot->qdone);
+                    vm_drain();
+                    uint32_t tl = ld_relaxed(&slot->qtail);
+                    if (dn >= tl)
+                        st = 1;
+                    L.tail_seen = tl;
+                } else {
+                    L.tail_seen = 0xffffffffu;
+                }
+                if (ld_relaxed(&a.gs->err))
+                    st = 2;
+                if (k == 0 && st == 0) { // waiting: bounded like every device wait
+                    uint64_t now = now_ticks();
+                    if (!idle0) {
+                        idle0 = now;
+                    } else if (now - idle0 > a.gs->timeout_ticks) {
+                        sync_fail(a.gs, 1);
+                        st = 2;
+                    }
+                } else {
+                    idle0 = 0;
+                }
+                L.hstate = st;
+            }
+            __syncthreads();
+            const uint32_t hs = L.hstate;
+            if (hs == 2)
+                return;
+            if (hs == 1 && pending && p >= L.tail_seen)
+                pending = false; // never to be written
+            if (!__syncthreads_or(pending ? 1 : 0)) {
+                if (hs == 1)
+                    return; // the phase is over
+                break;      // reservation done: reserve again
+            }
+            if (k == 0) {
+                // the counters all waiters poll share one cache line: back
+                // off so the polling does not crowd the workgroups still busy
+                for (uint32_t i = 0; i < backoff; i++)
+                    __builtin_amdgcn_s_sleep(8);
+                backoff = backoff < 16 ? backoff * 2 : 16;
+            } else {
+                backoff = 1;
+            }
+        }
     }
 }
 
@@ -627,8 +787,10 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             tr[0] = now_ticks();
         if (kind == PH_SCAN) {
             scan_slice<BLOCK, LQ, R>(L, a, level, phase);
-            if (a.mode != MODE_GPP)
+            if (a.mode != MODE_GPP) {
                 drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+                help_shared<BLOCK, LQ, R>(L, a, level, phase, edges);
+            }
         } else if (kind == PH_SHARED) {
             take_shared<BLOCK, LQ, R>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
         } else {
@@ -662,7 +824,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             if (L.live)
                 atomicOr(&s->flag, 1u);
         }
-        if (!grid_barrier(gs, gen))
+        if (!grid_barrier<false>(gs, gen))
             break;
         PhaseSlot *s = &gs->slot[phase & 1];
         uint64_t claims = ld_relaxed64(&s->sum0);
@@ -715,13 +877,8 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             continue;
         }
 
-        // bucket peel / PeelOne
+        // bucket peel / PeelOne: the phase consumed its own shared list
         lvl_claims += claims;
-        if (n_next > 0) {
-            kind = PH_SHARED;
-            n_prev = n_next;
-            continue;
-        }
         iters++;
         levels_run++;
         total += lvl_claims;
@@ -759,10 +916,15 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
 template <typename R>
 __global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uint64_t *done,
                                  uint32_t ndone, uint32_t *out, GridSync *gs, uint32_t nblocks,
-                                 uint64_t timeout, uint64_t *stats)
+                                 uint64_t timeout, uint64_t *stats, uint64_t *list0,
+                                 uint64_t *list1, uint32_t list_fill)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j = i; j < list_fill; j += stride) { // free slots (bucket / PeelOne)
+        list0[j] = KC_EMPTY;
+        list1[j] = KC_EMPTY;
+    }
     for (uint32_t v = i; v < V; v += stride) {
         deg[v] = (uint32_t)(rowptr[v + 1] - rowptr[v]);
         if (out)
@@ -781,6 +943,7 @@ __global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uin
             gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
             gs->slot[p].min0 = 0xffffffffu;
             gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
+            gs->slot[p].busy = nblocks;
         }
         for (int k = 0; k < KC_ST_COUNT; k++)
             stats[k] = 0;
@@ -819,9 +982,10 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.trace = p.trace;
     a.trace_cap = p.trace_cap;
     uint32_t ndone = (p.V + 63) / 64;
-    hipLaunchKernelGGL((peel_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V, p.deg,
+    hipLaunchKernelGGL((peel_init_kernel<R>), dim3(1024), dim3(256), 0, st, a.rowptr, p.V, p.deg,
                        p.done, ndone, p.mode == MODE_GPP ? p.out : nullptr, p.gs, p.grid,
-                       p.timeout_ticks, p.stats);
+                       p.timeout_ticks, p.stats, p.list0, p.list1,
+                       p.mode == MODE_GPP ? 0u : p.list_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
