@@ -493,7 +493,8 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         (s = ensure(g, g->kmax, 8, "kmax")))
         return s;
     uint32_t mode = method == KC_GPP ? 2 : method == KC_PEELONE ? 1 : 0;
-    if (mode == 2 && (s = ensure(g, g->out, (size_t)g->V * 4, "gpp out")))
+    // GPP: final cores; bucket peel: the live-vertex list of the high levels
+    if (mode != 1 && (s = ensure(g, g->out, (size_t)(g->V ? g->V : 1) * 4, "vertex list")))
         return s;
     kc::PeelLaunch p{};
     p.rowptr = g->rowptr;
@@ -508,6 +509,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.list_cap = cap;
     p.mode = mode;
     p.out = (uint32_t *)g->out.p;
+    p.live_thresh = g->V / 4; // switch to the live-vertex list below a quarter of V
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;
     p.grid = grid;

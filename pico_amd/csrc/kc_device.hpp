@@ -120,6 +120,8 @@ struct GridSync {
     uint32_t nblocks;
     uint64_t timeout_ticks;
     uint64_t t0;
+    uint32_t live_next; // reservations of the peel's live-vertex list
+    uint32_t pad;
     PhaseSlot slot[2];
 };
 

@@ -45,7 +45,8 @@ struct PeelLaunch {
     uint64_t *list0, *list1;
     uint32_t list_cap;
     uint32_t mode; // 0 bucket, 1 PeelOne sweeps, 2 GPP
-    uint32_t *out; // GPP final core
+    uint32_t *out; // GPP final core; bucket: the live-vertex list (V entries)
+    uint32_t live_thresh; // bucket: scan a live-vertex list once <= this many remain
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;

@@ -38,7 +38,8 @@ template <typename R> struct PeelArgs {
     uint64_t *list1;
     uint32_t list_cap;
     uint32_t mode;
-    uint32_t *out; // GPP: final core
+    uint32_t *out; // GPP: final core; bucket: live-vertex list
+    uint32_t live_thresh;
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional: KC_TRACE_WORDS per phase, see kc_engines.hpp
@@ -72,6 +73,9 @@ template <int BLOCK, int LQ> struct PeelLds {
     uint32_t hstate;
     uint32_t tail_seen;
     uint32_t hlen[BLOCK];
+    uint32_t lmode;       // scanning the live-vertex list instead of chunks
+    uint32_t lbase, lcnt; // this WG's region of the live-vertex list
+    uint32_t nkeep;
     uint32_t claims; // vertices claimed by this WG in the current phase
     uint32_t minv;   // min live degree seen by this WG's scan
     uint32_t live;   // any live vertex seen by the scan
@@ -244,6 +248,8 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
             uint32_t eff = old > level ? (old - level < c ? old - level : c) : 0u;
             if (c > eff)
                 atomicAdd(&a.deg[u], c - eff);
+            if (old - eff > level)
+                atomicMin(&L.minv, old - eff); // degree left above the level
             if (old > level && old - eff == level && a.mode == MODE_BUCKET) {
                 uint64_t bit = 1ull << (u & 63);
                 uint64_t pv = atomicOr((unsigned long long *)&a.done[u >> 6],
@@ -299,11 +305,14 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
             old[k] = hit[k] ? atomicSub(&a.deg[u[k]], 1u) : 0u;
         }
         bool claim[WALK_U];
+        uint32_t nmin = 0xffffffffu; // smallest degree left above the level
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
             claim[k] = false;
             if (!hit[k])
                 continue;
+            if (old[k] > level + 1 && old[k] - 1 < nmin)
+                nmin = old[k] - 1;
             if (a.mode == MODE_GPP) {
                 claim[k] = old[k] == level + 1;
             } else if (old[k] == level + 1) {
@@ -320,6 +329,9 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
 #pragma unroll
         for (int k = 0; k < WALK_U; k++)
             push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim[k], u[k]);
+        nmin = wave_min_u32(nmin);
+        if (lane_id() == 0 && nmin != 0xffffffffu)
+            atomicMin(&L.minv, nmin);
     }
     __syncthreads();
     if (a.mode != MODE_GPP) {
@@ -513,6 +525,116 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         if (lane == 0)
             L.nlive = out;
     }
+    __syncthreads();
+}
+
+// Bucket peel, once few vertices remain: every workgroup packs the live
+// vertices of its chunks into its own region of a list (one reservation of
+// the region, then each thread writes its chunk's vertices), and the scans of
+// later levels read and compact that region instead of whole chunks.
+template <int BLOCK, int LQ, typename R>
+__device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
+{
+    const bool use_list = L.use_list != 0;
+    const uint32_t n = use_list ? L.nlive : L.ncw;
+    const uint32_t nb = gridDim.x;
+    // per thread: a strided share of this WG's chunks
+    uint32_t mine = 0;
+    for (uint32_t j = threadIdx.x; j < n; j += BLOCK) {
+        uint32_t c = use_list ? L.chunks[j] : blockIdx.x + j * nb;
+        if (c == KC_DEAD)
+            continue;
+        mine += (uint32_t)__popcll(~ld_relaxed64(&a.done[c]));
+    }
+    uint32_t total = 0;
+    uint32_t off = block_excl_scan<BLOCK>(mine, &total);
+    if (threadIdx.x == 0) {
+        L.lbase = total ? atomicAdd(&a.gs->live_next, total) : 0u;
+        L.lcnt = total;
+        L.lmode = 1;
+    }
+    __syncthreads();
+    // Other workgroups may claim vertices meanwhile: a re-read claim word has
+    // at most the free bits counted above, so a thread writes at most `mine`
+    // vertices and pads the rest of its share with KC_DEAD (skipped by scans).
+    uint32_t pos = L.lbase + off, end = pos + mine;
+    for (uint32_t j = threadIdx.x; j < n && pos < end; j += BLOCK) {
+        uint32_t c = use_list ? L.chunks[j] : blockIdx.x + j * nb;
+        if (c == KC_DEAD)
+            continue;
+        uint64_t free = ~ld_relaxed64(&a.done[c]);
+        while (free && pos < end) {
+            uint32_t b = __ffsll((unsigned long long)free) - 1;
+            free &= free - 1;
+            a.out[pos++] = c * 64 + b;
+        }
+    }
+    while (pos < end)
+        a.out[pos++] = KC_DEAD;
+    __syncthreads();
+}
+
+template <int BLOCK, int LQ, typename R>
+__device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+                          uint32_t phase)
+{
+    constexpr uint32_t U = 4;
+    const uint32_t base = L.lbase, n = L.lcnt;
+    const uint32_t lane = lane_id();
+    uint32_t *lst = a.out + base;
+    uint32_t lmin = 0xffffffffu;
+    bool live_seen = false;
+    if (threadIdx.x == 0)
+        L.nkeep = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < n; b0 += BLOCK * U) {
+        uint32_t v[U], d[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            uint32_t i = b0 + k * BLOCK + threadIdx.x;
+            v[k] = i < n ? lst[i] : KC_DEAD;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++)
+            d[k] = v[k] != KC_DEAD ? ld_relaxed(&a.deg[v[k]]) : 0u;
+        __syncthreads(); // every read of this step is done before any write
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            bool cand = v[k] != KC_DEAD && d[k] == level;
+            bool keep = v[k] != KC_DEAD && d[k] > level;
+            if (keep && d[k] < lmin)
+                lmin = d[k];
+            live_seen |= cand || keep;
+            bool claim = false;
+            if (cand) {
+                uint64_t bit = 1ull << (v[k] & 63);
+                uint64_t pv = atomicOr((unsigned long long *)&a.done[v[k] >> 6],
+                                       (unsigned long long)bit);
+                claim = (pv & bit) == 0;
+            }
+            push_claims<BLOCK, LQ, R>(L, 0, a, phase, claim, v[k]);
+            uint64_t km = ballot(keep);
+            if (km) {
+                uint32_t leader = __ffsll((unsigned long long)km) - 1;
+                uint32_t pos = 0;
+                if (lane == leader)
+                    pos = atomicAdd(&L.nkeep, (uint32_t)__popcll(km));
+                pos = __shfl(pos, leader, KC_WAVE);
+                if (keep)
+                    lst[pos + lane_rank(km)] = v[k]; // below b0 + BLOCK*U: already read
+            }
+        }
+    }
+    lmin = wave_min_u32(lmin);
+    uint64_t lvm = ballot(live_seen);
+    if (lane == 0) {
+        atomicMin(&L.minv, lmin);
+        if (lvm)
+            L.live = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        L.lcnt = L.nkeep;
     __syncthreads();
 }
 
@@ -765,6 +887,9 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         L.cnt[1] = 0;
         L.pend[0] = 0;
         L.pend[1] = 0;
+        L.lmode = 0;
+        L.lbase = 0;
+        L.lcnt = 0;
     }
     for (uint32_t s = threadIdx.x; s < KC_AGG; s += BLOCK) {
         L.agg_key[s] = KC_DEAD;
@@ -786,7 +911,13 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         if (tr && blockIdx.x == 0 && threadIdx.x == 0)
             tr[0] = now_ticks();
         if (kind == PH_SCAN) {
-            scan_slice<BLOCK, LQ, R>(L, a, level, phase);
+            if (a.mode == MODE_BUCKET && !L.lmode && a.live_thresh &&
+                (uint64_t)a.V - total <= a.live_thresh)
+                build_live_list<BLOCK, LQ, R>(L, a);
+            if (L.lmode)
+                scan_list<BLOCK, LQ, R>(L, a, level, phase);
+            else
+                scan_slice<BLOCK, LQ, R>(L, a, level, phase);
             if (a.mode != MODE_GPP) {
                 drain<BLOCK, LQ, R>(L, a, level, phase, edges);
                 help_shared<BLOCK, LQ, R>(L, a, level, phase, edges);
@@ -889,14 +1020,15 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         if (a.mode == MODE_PEELONE) {
             if (lvl_claims == 0)
                 level++;
-        } else if (lvl_claims == 0) {
+        } else {
+            // mn is the smallest degree of any vertex still live: the scan's
+            // minimum over untouched vertices and every decrement's result
+            // that stayed above the level, so empty levels are never visited
             if (mn == 0xffffffffu || mn <= level) {
                 sync_fail(gs, 3);
                 break;
             }
             level = mn;
-        } else {
-            level++;
         }
         lvl_claims = 0;
         kind = PH_SCAN;
@@ -939,6 +1071,7 @@ __global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uin
         gs->nblocks = nblocks;
         gs->timeout_ticks = timeout;
         gs->t0 = 0;
+        gs->live_next = 0;
         for (int p = 0; p < 2; p++) {
             gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
             gs->slot[p].min0 = 0xffffffffu;
@@ -977,6 +1110,7 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.list_cap = p.list_cap;
     a.mode = p.mode;
     a.out = p.out;
+    a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
