@@ -207,7 +207,7 @@ __device__ __forceinline__ uint32_t block_sum(HLds<BLOCK> &L, uint32_t x)
 // Block-wide top-down search over HBM bins[1..top]; each thread reads four
 // consecutive bins per step (4*BLOCK bins in flight per step).
 template <int BLOCK>
-__device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t top,
+__device__ uint32_t block_hsearch(HLds<BLOCK> &L, uint32_t *bins, uint32_t top,
                                   uint32_t &sum)
 {
     constexpr int NW = BLOCK / KC_WAVE;
@@ -221,7 +221,8 @@ __device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             int64_t j = j0 - q;
-            b[q] = j >= 1 ? bins[j] : 0u;
+            // device-coherent: the bins were just built by atomics
+            b[q] = j >= 1 ? ld_relaxed(&bins[j]) : 0u;
             mine += b[q];
         }
         uint32_t incl = wave_incl_scan_u32(mine);
@@ -276,16 +277,19 @@ __device__ uint32_t block_hsearch(HLds<BLOCK> &L, const uint32_t *bins, uint32_t
 template <typename R, int BLOCK>
 __device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t *bins)
 {
+    // Only this workgroup touches these bins, and they are read back with
+    // device-coherent loads: completing the stores / atomics (vmcnt) orders
+    // them without an L2-flushing fence.
     for (uint32_t j = threadIdx.x; j <= cap; j += BLOCK)
-        bins[j] = 0;
-    __threadfence();
+        st_relaxed(&bins[j], 0u);
+    vm_drain();
     __syncthreads();
     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
     for (uint64_t i = s + threadIdx.x; i < e; i += BLOCK) {
         uint32_t x = a.core[a.col[i]];
         atomicAdd(&bins[x < cap ? x : cap], 1u);
     }
-    __threadfence();
+    vm_drain();
     __syncthreads();
 }
 

@@ -427,7 +427,10 @@ __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs
 {
     uint32_t lane = lane_id();
     uint32_t v = c * 64 + lane;
-    bool cand = live && (a.mode == MODE_GPP ? d <= level : d == level);
+    // An unclaimed vertex below the level is one at the level caught in the
+    // brief dip of a clamped decrement (subtract, then add back): it is a
+    // candidate like one read exactly at the level.
+    bool cand = live && d <= level;
     if (live && !cand)
         lmin = d < lmin ? d : lmin;
     uint64_t cm = ballot(cand);
@@ -597,11 +600,20 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
 #pragma unroll
         for (uint32_t k = 0; k < U; k++)
             d[k] = v[k] != KC_DEAD ? ld_relaxed(&a.deg[v[k]]) : 0u;
+        // below the level: dead if claimed, else a vertex at the level read
+        // during the dip of a clamped decrement
+        bool dead[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; k++) {
+            dead[k] = v[k] == KC_DEAD;
+            if (!dead[k] && d[k] < level)
+                dead[k] = (ld_relaxed64(&a.done[v[k] >> 6]) >> (v[k] & 63)) & 1ull;
+        }
         __syncthreads(); // every read of this step is done before any write
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) {
-            bool cand = v[k] != KC_DEAD && d[k] == level;
-            bool keep = v[k] != KC_DEAD && d[k] > level;
+            bool cand = !dead[k] && d[k] <= level;
+            bool keep = !dead[k] && d[k] > level;
             if (keep && d[k] < lmin)
                 lmin = d[k];
             live_seen |= cand || keep;
