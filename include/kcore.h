@@ -78,7 +78,10 @@ typedef struct kc_opts {
     uint32_t local_cap;  /* pending-edge cap of a workgroup's local frontier; the
                             excess is shared with every workgroup */
     uint32_t flags;      /* KC_OPT_* */
-    uint32_t reserved[4];
+    uint32_t agg_div;    /* bucket peels: read each target's degree before decrementing and
+                            sum hub decrements in LDS once at most V/agg_div vertices are
+                            live (0: 64; 0xffffffff: from the start) */
+    uint32_t reserved[3];
 } kc_opts;
 
 typedef struct kc_stats {

@@ -875,13 +875,15 @@ kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *bes
     for (uint32_t block : {256u, 512u, 1024u}) {
         if (peel) {
             for (uint32_t hub : {256u, 1024u, 4096u})
-                for (uint32_t lc : {0u, 4u, 64u}) { // pending edges per thread
-                    kc_opts o{};
-                    o.block = block;
-                    o.hub_split = hub;
-                    o.local_cap = lc ? lc * block : 0;
-                    cand.push_back(o);
-                }
+                for (uint32_t lc : {0u, 4u, 64u}) // pending edges per thread
+                    for (uint32_t ad : {32u, 128u}) {
+                        kc_opts o{};
+                        o.block = block;
+                        o.hub_split = hub;
+                        o.local_cap = lc ? lc * block : 0;
+                        o.agg_div = ad;
+                        cand.push_back(o);
+                    }
         } else {
             kc_opts o{};
             o.block = block;

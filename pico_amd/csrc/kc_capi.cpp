@@ -510,6 +510,9 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.mode = mode;
     p.out = (uint32_t *)g->out.p;
     p.live_thresh = g->V / 4; // switch to the live-vertex list below a quarter of V
+    // the dense core: pre-check + aggregation once at most V/agg_div remain
+    uint32_t agg_div = o.agg_div ? o.agg_div : 64;
+    p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : g->V / agg_div;
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;
     p.grid = grid;

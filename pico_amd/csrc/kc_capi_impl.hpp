@@ -76,6 +76,8 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
         o.hub_split = t.hub_split;
     if (!o.local_cap)
         o.local_cap = t.local_cap;
+    if (!o.agg_div)
+        o.agg_div = t.agg_div;
     return o;
 }
 

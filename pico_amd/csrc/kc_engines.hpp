@@ -47,6 +47,7 @@ struct PeelLaunch {
     uint32_t mode; // 0 bucket, 1 PeelOne sweeps, 2 GPP
     uint32_t *out; // GPP final core; bucket: the live-vertex list (V entries)
     uint32_t live_thresh; // bucket: scan a live-vertex list once <= this many remain
+    uint32_t agg_live;    // pre-check degrees and aggregate hub decrements once <= this many remain
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;

@@ -40,6 +40,7 @@ template <typename R> struct PeelArgs {
     uint32_t mode;
     uint32_t *out; // GPP: final core; bucket: live-vertex list
     uint32_t live_thresh;
+    uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional: KC_TRACE_WORDS per phase, see kc_engines.hpp
@@ -73,6 +74,7 @@ template <int BLOCK, int LQ> struct PeelLds {
     uint32_t hstate;
     uint32_t tail_seen;
     uint32_t hlen[BLOCK];
+    uint32_t precheck;    // read the target's degree first (and aggregate hub decrements)
     uint32_t lmode;       // scanning the live-vertex list instead of chunks
     uint32_t lbase, lcnt; // this WG's region of the live-vertex list
     uint32_t nkeep;
@@ -279,11 +281,17 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
                 u[k] = a.col[L.start[i] + (e - L.pre[i])];
             }
         }
+        // While many vertices are live, decrement without looking first (one
+        // dependent access less per edge; a decrement that undershoots is
+        // undone).  In the dense core, read the degree first: dead targets
+        // are skipped and decrements to vertices far above the level are
+        // summed in LDS (flush_agg) instead of piling atomics on the hubs.
+        const bool pc = L.precheck != 0;
         uint64_t pre[WALK_U];
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
-            pre[k] = 0;
-            if (val[k])
+            pre[k] = 0xffffffffull;
+            if (val[k] && (a.mode == MODE_GPP || pc))
                 // GPP needs the exact claim word; the clamp pre-check may
                 // read a stale degree (stale values only ever lie above)
                 pre[k] = a.mode == MODE_GPP ? ld_relaxed64(&a.done[u[k] >> 6])
@@ -299,7 +307,7 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
                 hit[k] = val[k] && (uint32_t)pre[k] > level;
                 // decrements to vertices well above the level are summed in
                 // LDS and applied once per walk (flush_agg)
-                if (hit[k] && (uint32_t)pre[k] > level + KC_AGG_MARGIN && agg_add(L, u[k]))
+                if (pc && hit[k] && (uint32_t)pre[k] > level + KC_AGG_MARGIN && agg_add(L, u[k]))
                     hit[k] = false;
             }
             old[k] = hit[k] ? atomicSub(&a.deg[u[k]], 1u) : 0u;
@@ -914,6 +922,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             L.live = 0;
             L.rounds = 0;
             L.ph_edges = 0;
+            L.precheck = (uint64_t)a.V - total <= a.agg_live;
         }
         __syncthreads();
         const uint32_t phase = gen;
@@ -1123,6 +1132,7 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.mode = p.mode;
     a.out = p.out;
     a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
+    a.agg_live = p.agg_live;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;

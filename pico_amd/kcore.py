@@ -37,7 +37,7 @@ class Opts(ctypes.Structure):
                 ("hub_split", ctypes.c_uint32), ("queue_cap", ctypes.c_uint32),
                 ("gpp_compat", ctypes.c_uint32), ("part_size", ctypes.c_uint32),
                 ("local_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("agg_div", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -513,7 +513,8 @@ class DeviceGraph:
         o = Opts()
         ms = ctypes.c_double(0)
         _check(lib().kc_autotune(self._h, m, reps, ctypes.byref(o), ctypes.byref(ms)))
-        return {"block": o.block, "hub_split": o.hub_split, "local_cap": o.local_cap}, ms.value
+        return {"block": o.block, "hub_split": o.hub_split, "local_cap": o.local_cap,
+                "agg_div": o.agg_div}, ms.value
 
     def __init__(self, ctx, handle, V):
         self.ctx = ctx
