@@ -130,12 +130,18 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
 }
 
 // Push vertices this wave's lanes claimed.  Bucket/PeelOne: to the local
-// buffer `buf` (spilling to the shared list when full), hubs as chunk items
-// on the shared list; counted as claims.  GPP: crossers go to the shared list
-// whole and are claimed when the next iteration retires them.
+// buffer `buf`, spilling to the shared list when it is full; counted as
+// claims.  Their degrees are not read here: staging the next round reads the
+// row offsets anyway, splits hubs and caps the round's edges there, which
+// keeps one dependent global load off every cascade step.  A claim at level
+// 0 is an isolated vertex and has nothing to walk.  GPP: crossers go to the
+// shared list whole and are claimed when the next iteration retires them.
+constexpr uint32_t KC_WHOLE = 0xffffffffu; // shared item: a whole vertex, split by its taker
+
 template <int BLOCK, int LQ, typename R>
 __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, const PeelArgs<R> &a,
-                                            uint32_t phase, bool claim, uint32_t u)
+                                            uint32_t phase, uint32_t level, bool claim,
+                                            uint32_t u)
 {
     uint64_t cm = ballot(claim);
     if (!cm)
@@ -145,46 +151,42 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
         push_shared(a, phase, claim, (uint64_t)u);
         return;
     }
-    if (lane == (uint32_t)(__ffsll((unsigned long long)cm) - 1))
+    uint32_t leader = __ffsll((unsigned long long)cm) - 1;
+    uint32_t base = 0;
+    if (lane == leader) {
         atomicAdd(&L.claims, (uint32_t)__popcll(cm));
-    uint32_t d0 = claim ? deg0(a.rowptr, u) : 0;
-    bool hub = claim && d0 > a.hub_split;
-    bool loc = claim && !hub && d0 > 0; // a degree-0 claim has nothing to walk
-    uint64_t lm = ballot(loc);
-    bool spill = false;
-    if (lm) {
-        // The local frontier holds at most local_cap pending edges; once the
-        // WG's backlog is that large, further claims go to the shared list so
-        // a wide cascade is spread over every workgroup.
-        uint32_t wsum = wave_sum_u32(loc ? d0 : 0u);
-        uint32_t leader = __ffsll((unsigned long long)lm) - 1;
-        uint32_t base = 0xffffffffu;
-        if (lane == leader) {
-            uint32_t old_e = atomicAdd(&L.pend[buf], wsum);
-            if (old_e < a.local_cap)
-                base = atomicAdd(&L.cnt[buf], (uint32_t)__popcll(lm));
-        }
-        base = __shfl(base, leader, KC_WAVE);
-        if (loc) {
-            uint32_t idx = base == 0xffffffffu ? 0xffffffffu : base + lane_rank(lm);
-            if (idx < (uint32_t)LQ)
-                L.lbuf[buf][idx] = u;
-            else
-                spill = true;
-        }
+        if (level != 0)
+            base = atomicAdd(&L.cnt[buf], (uint32_t)__popcll(cm));
     }
-    push_shared(a, phase, spill, (uint64_t)u);
-    if (hub) {
-        PhaseSlot *slot = &a.gs->slot[phase & 1];
-        uint64_t *list = (phase & 1) ? a.list1 : a.list0;
-        uint32_t nch = (d0 + a.hub_split - 1) / a.hub_split;
-        uint32_t base = atomicAdd(&slot->qtail, nch);
-        if (base + nch <= a.list_cap) {
-            for (uint32_t c = 0; c < nch; c++)
-                st_relaxed64(&list[base + c], (uint64_t)u | ((uint64_t)c << 32));
-        } else {
-            sync_fail(a.gs, 2);
-        }
+    if (level == 0)
+        return;
+    base = __shfl(base, leader, KC_WAVE);
+    bool spill = false;
+    if (claim) {
+        uint32_t idx = base + lane_rank(cm);
+        if (idx < (uint32_t)LQ)
+            L.lbuf[buf][idx] = u;
+        else
+            spill = true;
+    }
+    push_shared(a, phase, spill, (uint64_t)u | ((uint64_t)KC_WHOLE << 32));
+}
+
+// Push chunks [1, nch) of hub v (chunk 0 stays with the caller).
+template <typename R>
+__device__ __forceinline__ void push_hub_chunks(const PeelArgs<R> &a, uint32_t phase, uint32_t v,
+                                                uint32_t nch)
+{
+    if (nch <= 1)
+        return;
+    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    uint64_t *list = (phase & 1) ? a.list1 : a.list0;
+    uint32_t base = atomicAdd(&slot->qtail, nch - 1);
+    if (base + nch - 1 <= a.list_cap) {
+        for (uint32_t c = 1; c < nch; c++)
+            st_relaxed64(&list[base + c - 1], (uint64_t)v | ((uint64_t)c << 32));
+    } else {
+        sync_fail(a.gs, 2);
     }
 }
 
@@ -259,7 +261,7 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                 claim = (pv & bit) == 0;
             }
         }
-        push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim, u);
+        push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, level, claim, u);
     }
 }
 
@@ -336,7 +338,7 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         }
 #pragma unroll
         for (int k = 0; k < WALK_U; k++)
-            push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, claim[k], u[k]);
+            push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, level, claim[k], u[k]);
         nmin = wave_min_u32(nmin);
         if (lane_id() == 0 && nmin != 0xffffffffu)
             atomicMin(&L.minv, nmin);
@@ -397,16 +399,37 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
             break;
         for (uint32_t base = 0; base < n; base += BLOCK) {
             uint32_t m = n - base < (uint32_t)BLOCK ? n - base : (uint32_t)BLOCK;
-            uint32_t len = 0;
+            uint32_t len = 0, v = 0;
             if (threadIdx.x < m) {
-                uint32_t v = L.lbuf[cur][base + threadIdx.x];
+                v = L.lbuf[cur][base + threadIdx.x];
                 R s = a.rowptr[v];
-                len = (uint32_t)(a.rowptr[v + 1] - s);
+                uint32_t d = (uint32_t)(a.rowptr[v + 1] - s);
+                // a hub walks its first hub_split edges here; every workgroup
+                // can take the rest from the shared list
+                if (d > a.hub_split) {
+                    push_hub_chunks(a, phase, v, (d + a.hub_split - 1) / a.hub_split);
+                    d = a.hub_split;
+                }
+                len = d;
                 L.start[threadIdx.x] = (uint64_t)s;
             }
-            edges += len;
             stage_prefix<BLOCK, LQ>(L, m, len);
-            walk_items<BLOCK, LQ, R>(L, a, level, phase, m, cur ^ 1);
+            // At most local_cap edges per round stay here; the items past the
+            // cap (by the prefix, a suffix of the batch) go to the shared list
+            // for every workgroup: a wide cascade spreads over the grid.
+            uint32_t keep = m;
+            if (L.pre[m] > a.local_cap) {
+                keep = find_item64(L.pre, m, a.local_cap) + 1;
+                if (keep < 1)
+                    keep = 1;
+            }
+            push_shared(a, phase, threadIdx.x >= keep && threadIdx.x < m, (uint64_t)v);
+            if (keep < m) {
+                __syncthreads();
+                stage_prefix<BLOCK, LQ>(L, keep, threadIdx.x < keep ? len : 0u);
+            }
+            edges += threadIdx.x < keep ? len : 0u;
+            walk_items<BLOCK, LQ, R>(L, a, level, phase, keep, cur ^ 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -457,7 +480,7 @@ __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs
         nret += claim ? 1u : 0u;
         push_shared(a, phase, claim, (uint64_t)v);
     } else {
-        push_claims<BLOCK, LQ, R>(L, 0, a, phase, claim, v);
+        push_claims<BLOCK, LQ, R>(L, 0, a, phase, level, claim, v);
     }
 }
 
@@ -632,7 +655,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                                        (unsigned long long)bit);
                 claim = (pv & bit) == 0;
             }
-            push_claims<BLOCK, LQ, R>(L, 0, a, phase, claim, v[k]);
+            push_claims<BLOCK, LQ, R>(L, 0, a, phase, level, claim, v[k]);
             uint64_t km = ballot(keep);
             if (km) {
                 uint32_t leader = __ffsll((unsigned long long)km) - 1;
@@ -794,9 +817,15 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
             const uint32_t k = L.claims_tmp; // items taken this round
             if (have) {
                 uint32_t v = (uint32_t)it;
+                uint32_t c = (uint32_t)(it >> 32);
                 R rs = a.rowptr[v];
                 uint32_t d = (uint32_t)(a.rowptr[v + 1] - rs);
-                uint64_t off = (uint64_t)(uint32_t)(it >> 32) * a.hub_split;
+                if (c == KC_WHOLE) { // a spilled claim: split it here if it is a hub
+                    if (d > a.hub_split)
+                        push_hub_chunks(a, phase, v, (d + a.hub_split - 1) / a.hub_split);
+                    c = 0;
+                }
+                uint64_t off = (uint64_t)c * a.hub_split;
                 uint32_t rem = (uint32_t)(d - off);
                 L.start[idx] = (uint64_t)rs + off;
                 L.hlen[idx] = rem < a.hub_split ? rem : a.hub_split;
