@@ -109,8 +109,13 @@ struct PhaseSlot {
     uint32_t qtail; // items pushed to the shared list of this phase
     uint32_t qhead; // grab counter for reading that list
     uint32_t flag;  // engine-defined OR
-    uint32_t busy;  // workgroups still in their own work this phase (starts at nblocks)
-    uint32_t qdone; // shared-list items fully processed this phase
+    uint32_t pad0;
+    uint32_t pad1;
+    // Work list of the peel's bucket / PeelOne modes: items pushed (high 32
+    // bits) and pending work (low 32: workgroups still in their own work,
+    // which starts at nblocks, plus pushed items not yet fully processed).
+    // One word, so a single atomic load tells whether the phase is over.
+    uint64_t qword;
 };
 
 struct GridSync {
@@ -127,8 +132,7 @@ struct GridSync {
 
 __device__ __forceinline__ void slot_clear(PhaseSlot *s, uint32_t nblocks)
 {
-    st_relaxed(&s->busy, nblocks);
-    st_relaxed(&s->qdone, 0u);
+    __hip_atomic_store(&s->qword, (uint64_t)nblocks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s->sum0, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s->sum1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s->sum2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -31,8 +31,10 @@ enum {
 //  [7] phase kind
 //  [8] most local drain rounds in one WG   [9] edges walked (all WGs)
 //  [10] WG 0 drain rounds                  [11] most edges walked by one WG
+//  peel only: [12] latest end of a WG's scan  [13] latest end of a WG's own
+//  cascade (before it helps with shared work)  [14] latest WG phase start
 // Times are wall_clock64 ticks.
-enum { KC_TRACE_WORDS = 12 };
+enum { KC_TRACE_WORDS = 16 };
 
 struct PeelLaunch {
     const void *rowptr; // u32[V+1] or u64[V+1] (wide)

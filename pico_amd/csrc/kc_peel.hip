@@ -117,8 +117,15 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
     uint64_t *list = (phase & 1) ? a.list1 : a.list0;
     uint32_t leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
-    if (lane == leader)
-        base = atomicAdd(&slot->qtail, (uint32_t)__popcll(m));
+    if (lane == leader) {
+        uint32_t k = (uint32_t)__popcll(m);
+        if (a.mode == MODE_GPP)
+            base = atomicAdd(&slot->qtail, k);
+        else // tail and pending work move together
+            base = (uint32_t)(atomicAdd((unsigned long long *)&slot->qword,
+                                        ((unsigned long long)k << 32) | k) >>
+                              32);
+    }
     base = __shfl(base, leader, KC_WAVE);
     if (want) {
         uint32_t p = base + lane_rank(m);
@@ -181,7 +188,9 @@ __device__ __forceinline__ void push_hub_chunks(const PeelArgs<R> &a, uint32_t p
         return;
     PhaseSlot *slot = &a.gs->slot[phase & 1];
     uint64_t *list = (phase & 1) ? a.list1 : a.list0;
-    uint32_t base = atomicAdd(&slot->qtail, nch - 1);
+    uint64_t k = nch - 1;
+    uint32_t base =
+        (uint32_t)(atomicAdd((unsigned long long *)&slot->qword, (k << 32) | k) >> 32);
     if (base + nch - 1 <= a.list_cap) {
         for (uint32_t c = 1; c < nch; c++)
             st_relaxed64(&list[base + c - 1], (uint64_t)v | ((uint64_t)c << 32));
@@ -750,10 +759,11 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 // then write them; takers reserve slots at the head with an atomicAdd that
 // may run past the tail (those slots are filled by later pushes, or never).
 // A reserved slot is taken once it leaves KC_EMPTY and is emptied again.
-// Termination: `busy` counts workgroups still in their own work, `qdone` the
-// items whose processing (and the pushes it caused) is complete; once busy
-// is 0 and qdone, read before the tail, equals the tail, nothing can be
-// pushed any more and every slot at or past the tail is dropped.  Ready items are always processed
+// Termination: the low half of slot->qword counts the workgroups still in
+// their own work plus the pushed items not yet fully processed (a push adds
+// to it together with the tail; an item is subtracted only after the pushes
+// its processing caused).  When it reads 0, nothing can be pushed any more
+// and every reserved slot at or past the tail is dropped.  Ready items are always processed
 // before waiting on the rest of a reservation, so no workgroup sits on work
 // another one waits for.
 template <int BLOCK, int LQ, typename R>
@@ -766,28 +776,69 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
     // Every access the protocol relies on is device-coherent (atomics, and
     // atomic loads / stores of list slots), so ordering needs only that a
     // WG's earlier accesses have completed before thread 0 moves a counter.
+    unsigned long long *qw = (unsigned long long *)&slot->qword;
     vm_drain();
     __syncthreads();
-    if (threadIdx.x == 0)
-        atomicSub(&slot->busy, 1u); // this WG's own work is over
+    if (threadIdx.x == 0) {
+        // this WG's own work is over; if that was the last pending work, the
+        // phase is over and nobody needs to look again
+        unsigned long long old = atomicSub(qw, 1ull);
+        L.hstate = (uint32_t)old == 1u ? 1u : 0u;
+    }
+    __syncthreads();
+    if (L.hstate == 1)
+        return;
+    uint32_t backoff = 1;
+    uint64_t wait0 = 0; // thread 0: start of the current wait for items
     for (;;) {
-        // ---- reserve a range of slots
+        // ---- wait until items are visible (or the phase is over), then
+        // reserve a range of slots
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t t = ld_relaxed(&slot->qtail), h = ld_relaxed(&slot->qhead);
-            uint32_t n = t > h ? (t - h + nb - 1) / nb : 1u;
-            n = n > (uint32_t)BLOCK ? (uint32_t)BLOCK : n;
-            L.grab = atomicAdd(&slot->qhead, n);
-            L.hn = n;
+            uint32_t st = 0; // 0 reserved, 1 phase over, 2 error, 3 nothing yet
+            uint64_t w = ld_relaxed64(&slot->qword);
+            uint32_t t = (uint32_t)(w >> 32), h = ld_relaxed(&slot->qhead);
+            if ((uint32_t)w == 0) {
+                st = 1;
+            } else if (ld_relaxed(&a.gs->err)) {
+                st = 2;
+            } else if (t > h) {
+                uint32_t n = (t - h + nb - 1) / nb;
+                n = n > (uint32_t)BLOCK ? (uint32_t)BLOCK : n;
+                L.grab = atomicAdd(&slot->qhead, n);
+                L.hn = n;
+                wait0 = 0;
+            } else {
+                st = 3; // bounded like every device wait
+                uint64_t now = now_ticks();
+                if (!wait0) {
+                    wait0 = now;
+                } else if (now - wait0 > a.gs->timeout_ticks) {
+                    sync_fail(a.gs, 1);
+                    st = 2;
+                }
+            }
+            L.hstate = st;
         }
         __syncthreads();
+        if (L.hstate == 1 || L.hstate == 2)
+            return;
+        if (L.hstate == 3) {
+            // the counters all waiters poll share one cache line: back off
+            // so the polling does not crowd the workgroups still busy
+            for (uint32_t i = 0; i < backoff; i++)
+                __builtin_amdgcn_s_sleep(8);
+            backoff = backoff < 16 ? backoff * 2 : 16;
+            continue;
+        }
+        backoff = 1;
         const uint32_t s = L.grab, n = L.hn;
         const uint32_t p = s + threadIdx.x;
         // slots past the list's capacity can never be written (a push there
         // reports the overflow itself)
         bool pending = threadIdx.x < n && p < a.list_cap;
-        uint64_t idle0 = 0;   // thread 0: start of the current wait
-        uint32_t backoff = 1; // idle polls sleep longer each time (up to ~4 us)
+        uint64_t idle0 = 0; // thread 0: start of the current wait
+        uint32_t ib = 1;    // idle polls sleep longer each time (up to ~4 us)
         for (;;) {
             // ---- take what is ready
             uint64_t it = KC_EMPTY;
@@ -840,25 +891,15 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
                 vm_drain(); // the pushes these items caused come first
                 __syncthreads();
                 if (threadIdx.x == 0)
-                    atomicAdd(&slot->qdone, k);
+                    atomicSub(qw, (unsigned long long)k);
             }
             // ---- anything left in the reservation?
             if (threadIdx.x == 0) {
                 uint32_t st = 0; // 0 go on, 1 phase over, 2 error
-                if (ld_relaxed(&slot->busy) == 0) {
-                    // done first, tail second: if every item pushed by the
-                    // time `done` was read had been processed, and the tail
-                    // has not moved since, nothing is in flight
-                    vm_drain();
-                    uint32_t dn = ld_relaxed(&slot->qdone);
-                    vm_drain();
-                    uint32_t tl = ld_relaxed(&slot->qtail);
-                    if (dn >= tl)
-                        st = 1;
-                    L.tail_seen = tl;
-                } else {
-                    L.tail_seen = 0xffffffffu;
-                }
+                uint64_t w = ld_relaxed64(&slot->qword);
+                if ((uint32_t)w == 0)
+                    st = 1;
+                L.tail_seen = (uint32_t)(w >> 32);
                 if (ld_relaxed(&a.gs->err))
                     st = 2;
                 if (k == 0 && st == 0) { // waiting: bounded like every device wait
@@ -886,13 +927,11 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
                 break;      // reservation done: reserve again
             }
             if (k == 0) {
-                // the counters all waiters poll share one cache line: back
-                // off so the polling does not crowd the workgroups still busy
-                for (uint32_t i = 0; i < backoff; i++)
+                for (uint32_t i = 0; i < ib; i++)
                     __builtin_amdgcn_s_sleep(8);
-                backoff = backoff < 16 ? backoff * 2 : 16;
+                ib = ib < 16 ? ib * 2 : 16;
             } else {
-                backoff = 1;
+                ib = 1;
             }
         }
     }
@@ -958,8 +997,12 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         uint64_t *tr = (a.trace && phase < a.trace_cap) ? a.trace + (uint64_t)phase * KC_TRACE_WORDS
                                                         : nullptr;
         const uint64_t edges_before = edges;
-        if (tr && blockIdx.x == 0 && threadIdx.x == 0)
-            tr[0] = now_ticks();
+        if (tr && threadIdx.x == 0) {
+            uint64_t t = now_ticks();
+            if (blockIdx.x == 0)
+                tr[0] = t;
+            atomicMax((unsigned long long *)&tr[14], (unsigned long long)t);
+        }
         if (kind == PH_SCAN) {
             if (a.mode == MODE_BUCKET && !L.lmode && a.live_thresh &&
                 (uint64_t)a.V - total <= a.live_thresh)
@@ -968,8 +1011,12 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
                 scan_list<BLOCK, LQ, R>(L, a, level, phase);
             else
                 scan_slice<BLOCK, LQ, R>(L, a, level, phase);
+            if (tr && threadIdx.x == 0)
+                atomicMax((unsigned long long *)&tr[12], (unsigned long long)now_ticks());
             if (a.mode != MODE_GPP) {
                 drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+                if (tr && threadIdx.x == 0)
+                    atomicMax((unsigned long long *)&tr[13], (unsigned long long)now_ticks());
                 help_shared<BLOCK, LQ, R>(L, a, level, phase, edges);
             }
         } else if (kind == PH_SHARED) {
@@ -1009,7 +1056,8 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             break;
         PhaseSlot *s = &gs->slot[phase & 1];
         uint64_t claims = ld_relaxed64(&s->sum0);
-        uint32_t n_next = ld_relaxed(&s->qtail);
+        uint32_t n_next = a.mode == MODE_GPP ? ld_relaxed(&s->qtail)
+                                             : (uint32_t)(ld_relaxed64(&s->qword) >> 32);
         uint32_t mn = ld_relaxed(&s->min0);
         uint32_t live = ld_relaxed(&s->flag);
         if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1126,7 +1174,7 @@ __global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uin
             gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
             gs->slot[p].min0 = 0xffffffffu;
             gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
-            gs->slot[p].busy = nblocks;
+            gs->slot[p].qword = nblocks;
         }
         for (int k = 0; k < KC_ST_COUNT; k++)
             stats[k] = 0;

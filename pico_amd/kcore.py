@@ -793,7 +793,7 @@ def device_trace(dgraph, max_phases=1 << 16):
     """Per-phase trace of the last decompose(..., flags=KC_OPT_TRACE):
     a dict of numpy arrays with times in milliseconds."""
     L = _trace_sig()
-    buf = np.zeros((max_phases, 12), np.uint64)
+    buf = np.zeros((max_phases, 16), np.uint64)
     tpm = ctypes.c_uint64(0)
     n = L.kc_graph_trace(dgraph._h, buf.ctypes.data_as(_vp), max_phases, ctypes.byref(tpm))
     t = buf[:n].astype(np.float64)
@@ -806,6 +806,8 @@ def device_trace(dgraph, max_phases=1 << 16):
         "shared": buf[:n, 6].copy(), "kind": buf[:n, 7].copy(),
         "max_rounds": buf[:n, 8].copy(), "edges": buf[:n, 9].copy(),
         "wg0_rounds": buf[:n, 10].copy(), "max_wg_edges": buf[:n, 11].copy(),
+        "scan_done_ms": (t[:, 12] - t0) / f, "own_done_ms": (t[:, 13] - t0) / f,
+        "last_start_ms": (t[:, 14] - t0) / f,
     }
 
 
