@@ -828,7 +828,7 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
             // so the polling does not crowd the workgroups still busy
             for (uint32_t i = 0; i < backoff; i++)
                 __builtin_amdgcn_s_sleep(8);
-            backoff = backoff < 16 ? backoff * 2 : 16;
+            backoff = backoff < 4 ? backoff * 2 : 4;
             continue;
         }
         backoff = 1;
@@ -929,7 +929,7 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
             if (k == 0) {
                 for (uint32_t i = 0; i < ib; i++)
                     __builtin_amdgcn_s_sleep(8);
-                ib = ib < 16 ? ib * 2 : 16;
+                ib = ib < 4 ? ib * 2 : 4;
             } else {
                 ib = 1;
             }
