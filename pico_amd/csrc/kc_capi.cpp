@@ -392,7 +392,7 @@ void kc_graph_free(kc_graph *g)
     }
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
-                      &g->trace, &g->col_ds, &g->nd_ds})
+                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds})
         if (b->p)
             hipFree(b->p);
     if (g->ev0)
@@ -610,13 +610,18 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     if (use_ds && !g->degsorted) {
         size_t nb = (size_t)g->nnz * 4;
         if ((s = ensure(g, g->col_ds, nb, "ordered adjacency")) ||
-            (s = ensure(g, g->nd_ds, nb, "neighbour degrees")))
+            (s = ensure(g, g->nd_ds, nb, "neighbour degrees")) ||
+            (g->nnz + g->V < 0xffffffffull &&
+             (s = ensure(g, g->hb_ds, nb, "neighbour histogram bases"))))
             return s;
         double tp = now_ms();
         // the histogram buffer (>= nnz words) is free scratch before the run
         hipError_t pe = kc::build_degree_sorted(g->rowptr, g->wide, g->col, g->V, g->nnz,
                                                 (uint32_t *)g->col_ds.p, (uint32_t *)g->nd_ds.p,
                                                 (uint32_t *)g->histo.p, c->stream);
+        if (pe == hipSuccess && g->nnz + g->V < 0xffffffffull)
+            pe = kc::build_nbr_hbase(g->rowptr, g->wide, (const uint32_t *)g->col_ds.p, g->nnz,
+                                     (uint32_t *)g->hb_ds.p, c->stream);
         if (pe == hipSuccess)
             pe = hipStreamSynchronize(c->stream);
         if (pe != hipSuccess)
@@ -642,6 +647,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.chunk_cap = (uint32_t)chunk_cap;
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
     p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
+    p.hb_ds = use_ds && g->nnz + g->V < 0xffffffffull ? (const uint32_t *)g->hb_ds.p : nullptr;
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;
     p.grid = grid;

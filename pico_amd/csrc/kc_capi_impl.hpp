@@ -54,7 +54,7 @@ struct kc_graph {
     uint64_t bytes = 0;
     // method scratch, allocated on first use and kept for reuse
     DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
-    DevBuf col_ds, nd_ds; // degree-ordered adjacency (built on first HistoCore run)
+    DevBuf col_ds, nd_ds, hb_ds; // degree-ordered adjacency (built on first HistoCore run)
     bool degsorted = false;
     uint32_t trace_phases = 0;
     const uint32_t *result = nullptr;

@@ -78,6 +78,7 @@ struct HindexLaunch {
     uint32_t chunk_cap;
     const uint32_t *col_ds; // degree-ordered adjacency, or null
     const uint32_t *nd_ds;
+    const uint32_t *hb_ds;  // histogram base of each col_ds neighbour, or null
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;
@@ -174,6 +175,8 @@ uint32_t batch_lds_vertices(uint32_t block);
 
 // Rows re-ordered by neighbour degree, largest first (kc_prep.hip); nd_out
 // receives the neighbour degrees; nd_tmp is nnz words of scratch.
+hipError_t build_nbr_hbase(const void *rowptr, bool wide, const uint32_t *adj, uint64_t nnz,
+                           uint32_t *hb, hipStream_t st);
 hipError_t build_degree_sorted(const void *rowptr, bool wide, const uint32_t *col, uint32_t V,
                                uint64_t nnz, uint32_t *col_out, uint32_t *nd_out,
                                uint32_t *nd_tmp, hipStream_t st);

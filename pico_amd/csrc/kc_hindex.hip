@@ -52,13 +52,26 @@ template <typename R> struct HArgs {
     uint32_t chunk_cap;
     const uint32_t *col_ds; // degree-ordered adjacency (kc_prep.hip) or null
     const uint32_t *nd_ds;  // neighbour degrees beside col_ds
+    const uint32_t *hb_ds;  // neighbour histogram bases beside col_ds (nnz+V < 2^32), or null
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
     uint32_t trace_cap;
 };
 
+// HistoCore update phase: per-workgroup aggregation of the histogram
+// updates aimed at hubs (whose bins every changed low-core vertex hits).
+constexpr uint32_t KC_HAGG_LOG2 = 11;
+constexpr uint32_t KC_HAGG = 1u << KC_HAGG_LOG2;
+constexpr uint32_t KC_HAGG_PROBES = 8;
+constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggregated
+constexpr uint32_t KC_HEMPTY = 0xffffffffu;
+
 template <int BLOCK> struct HLds {
+    uint32_t agg_key[KC_HAGG]; // histogram word index
+    int32_t agg_d[KC_HAGG];    // summed change
+    uint32_t agg_u[KC_HAGG];   // owner vertex
+    uint32_t agg_cu[KC_HAGG];  // owner's core when the word is its top bin, else KC_HEMPTY
     uint32_t bins[BLOCK / KC_WAVE][KC_LBINS];
     uint32_t bigq[KC_BIGQ];
     uint32_t nbig;
@@ -73,6 +86,27 @@ template <int BLOCK> struct HLds {
 template <typename R> __device__ __forceinline__ uint64_t hbase(const R *rp, uint32_t v)
 {
     return (uint64_t)rp[v] + v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ bool hagg_add(HLds<BLOCK> &L, uint32_t word, int32_t d, uint32_t u,
+                                         uint32_t cu_top)
+{
+    uint32_t h = (word * 2654435761u) >> (32 - KC_HAGG_LOG2);
+#pragma unroll
+    for (uint32_t p = 0; p < KC_HAGG_PROBES; p++) {
+        uint32_t s = (h + p) & (KC_HAGG - 1);
+        uint32_t k = atomicCAS(&L.agg_key[s], KC_HEMPTY, word);
+        if (k == KC_HEMPTY || k == word) {
+            atomicAdd(&L.agg_d[s], d);
+            if (k == KC_HEMPTY) { // word -> (u, top-ness) is a function: any writer agrees
+                L.agg_u[s] = u;
+                L.agg_cu[s] = cu_top;
+            }
+            return true;
+        }
+    }
+    return false;
 }
 
 // Wave-aggregated append to list `lst` whose counter is this phase's slot.
@@ -399,6 +433,11 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     uint32_t n_act_prev = 0;
     if (a.trace && blockIdx.x == 0 && threadIdx.x == 0)
         a.trace[0] = now_ticks();
+    for (uint32_t sl = threadIdx.x; sl < KC_HAGG; sl += BLOCK) {
+        L.agg_key[sl] = KC_HEMPTY;
+        L.agg_d[sl] = 0;
+    }
+    __syncthreads();
 
     // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
     //      (histo_kernel, HistoCore.cu:15-75)
@@ -646,18 +685,52 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         uint32_t u = 0;
                         if (i < e) {
                             u = adj[i];
+                            uint64_t bu = a.hb_ds ? (uint64_t)a.hb_ds[i] : 0;
+                            bool hub = a.hb_ds && a.nd_ds[i] > KC_HUB_AGG;
                             uint32_t cu = a.core[u];
                             if (nc < cu) {
-                                uint64_t bu = hbase(a.rowptr, u);
+                                if (!a.hb_ds)
+                                    bu = hbase(a.rowptr, u);
                                 uint32_t sub = cu < oc ? cu : oc;
-                                uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
-                                atomicAdd(&a.histo[bu + nc], 1u);
-                                flag = oc >= cu && c == cu;
+                                // a hub's updates are summed per workgroup and
+                                // applied when the phase ends (flush below)
+                                if (!(hub && hagg_add(L, (uint32_t)(bu + sub), -1, u,
+                                                      sub == cu ? cu : KC_HEMPTY))) {
+                                    uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
+                                    flag = oc >= cu && c == cu;
+                                }
+                                if (!(hub && hagg_add(L, (uint32_t)(bu + nc), 1, u, KC_HEMPTY)))
+                                    atomicAdd(&a.histo[bu + nc], 1u);
                             }
                         }
                         hpush(gs, phase, f_list, cap_list, flag, u);
                     }
                 }
+            }
+            // Apply the summed hub updates.  A top bin only ever decreases in
+            // this phase, so the one subtraction (direct or summed, from any
+            // workgroup) that carries it from >= core to < core flags the
+            // owner, as the unit decrement landing on core does.
+            __syncthreads();
+            for (uint32_t s0 = 0; s0 < KC_HAGG; s0 += BLOCK) {
+                uint32_t sl = s0 + threadIdx.x;
+                uint32_t w = L.agg_key[sl];
+                bool flag = false;
+                uint32_t u = 0;
+                if (w != KC_HEMPTY) {
+                    int32_t d = L.agg_d[sl];
+                    if (d != 0) {
+                        uint32_t old = atomicAdd(&a.histo[w], (uint32_t)d);
+                        uint32_t cu = L.agg_cu[sl];
+                        if (cu != KC_HEMPTY && d < 0) {
+                            u = L.agg_u[sl];
+                            flag = old >= cu && old - (uint32_t)(-d) < cu;
+                        }
+                    }
+                    L.agg_key[sl] = KC_HEMPTY;
+                    L.agg_d[sl] = 0;
+                }
+                hpush(gs, phase, f_list, cap_list, flag, u);
             }
             if (!hbar(a, gs, gen, round, 2))
                 goto out;
@@ -849,6 +922,7 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     a.chunk_cap = p.chunk_cap;
     a.col_ds = p.col_ds;
     a.nd_ds = p.nd_ds;
+    a.hb_ds = p.hb_ds;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;

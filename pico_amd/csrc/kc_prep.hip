@@ -56,6 +56,33 @@ static hipError_t degsort_t(const R *rowptr, const uint32_t *col, uint32_t V, ui
     return e != hipSuccess ? e : e2;
 }
 
+// hb[i] = rowptr[adj[i]] + adj[i]: where neighbour adj[i]'s histogram starts
+// (HistoCore's layout), so an update reaches it without a dependent load
+template <typename R>
+__global__ void gather_nbr_hbase(const R *rowptr, const uint32_t *adj, uint64_t nnz, uint32_t *hb)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < nnz; i += stride) {
+        uint32_t u = adj[i];
+        hb[i] = (uint32_t)(rowptr[u] + u);
+    }
+}
+
+hipError_t build_nbr_hbase(const void *rowptr, bool wide, const uint32_t *adj, uint64_t nnz,
+                           uint32_t *hb, hipStream_t st)
+{
+    if (nnz == 0)
+        return hipSuccess;
+    if (wide)
+        hipLaunchKernelGGL(gather_nbr_hbase<uint64_t>, dim3(2048), dim3(256), 0, st,
+                           (const uint64_t *)rowptr, adj, nnz, hb);
+    else
+        hipLaunchKernelGGL(gather_nbr_hbase<uint32_t>, dim3(2048), dim3(256), 0, st,
+                           (const uint32_t *)rowptr, adj, nnz, hb);
+    return hipGetLastError();
+}
+
 hipError_t build_degree_sorted(const void *rowptr, bool wide, const uint32_t *col, uint32_t V,
                                uint64_t nnz, uint32_t *col_out, uint32_t *nd_out,
                                uint32_t *nd_tmp, hipStream_t st)
