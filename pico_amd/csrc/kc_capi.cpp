@@ -573,7 +573,9 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
 static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     kc_ctx *c = g->ctx;
-    uint32_t block = o.block ? o.block : 512;
+    // HistoCore's update phase favours more, smaller workgroups (each with
+    // its own hub aggregation table); the others favour 512
+    uint32_t block = o.block ? o.block : (method == KC_HISTO ? 256 : 512);
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
     int occ = kc::hindex_occupancy(block, g->wide);
