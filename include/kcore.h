@@ -66,6 +66,8 @@ typedef enum kc_method {
 /* kc_opts.flags bits */
 #define KC_OPT_TRACE 1u       /* record a per-phase device trace (kc_graph_trace) */
 #define KC_OPT_NO_DEGSORT 2u  /* HistoCore: walk the adjacency in input order */
+#define KC_OPT_COUNT 4u       /* peel methods: run the variant that counts its atomics
+                                 (kc_stats.atomics); a little slower, so off by default */
 
 /* Launch options; zero fields select defaults (or the autotuner's pick). */
 typedef struct kc_opts {
@@ -102,7 +104,10 @@ typedef struct kc_stats {
                          degree-ordered adjacency), outside device_ms */
     uint64_t changed_degree; /* HistoCore: degrees of the changed vertices summed over
                                 rounds (the adjacency the reference's update walks) */
-    uint64_t reserved[1];
+    uint64_t atomics; /* atomic read-modify-writes the kernel issued on graph-sized
+                         arrays: degrees and claim bitmap (peel, with KC_OPT_COUNT;
+                         0 without), histograms and marks (h-index, always); queue
+                         counters are not included */
 } kc_stats;
 
 KC_API const char *kc_last_error(void);

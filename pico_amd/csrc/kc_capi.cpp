@@ -519,6 +519,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.block = block;
     p.wide = g->wide;
     p.gpp_compat = o.gpp_compat != 0;
+    p.count_atomics = (o.flags & KC_OPT_COUNT) != 0;
     p.timeout_ticks = c->clock_khz * 1000ull * 20ull;
     g->trace_phases = 0;
     if (o.flags & KC_OPT_TRACE) {
@@ -559,6 +560,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     st.levels_run = sv[kc::KC_ST_LEVELS_RUN];
     st.phases = sv[kc::KC_ST_PHASES];
     st.edges_walked = sv[kc::KC_ST_EDGES];
+    st.atomics = sv[kc::KC_ST_ATOMICS];
     st.grid = grid;
     st.block = block;
     if (p.trace)
@@ -695,6 +697,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     st.edges_walked = sv[kc::KC_ST_EDGES];
     st.changed = sv[kc::KC_ST_CHANGED];
     st.changed_degree = sv[kc::KC_ST_CHG_DEG];
+    st.atomics = sv[kc::KC_ST_ATOMICS];
     if (p.trace)
         g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     st.grid = grid;

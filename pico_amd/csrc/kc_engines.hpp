@@ -21,7 +21,9 @@ enum {
     KC_ST_CHANGED = 5,    // h-index: total core-value changes
     KC_ST_ERR = 6,
     KC_ST_CHG_DEG = 7,    // h-index: degrees of changed vertices, summed
-    KC_ST_COUNT = 8
+    KC_ST_ATOMICS = 8,    // atomics issued on graph-sized arrays (degrees, claim
+                          // bitmap, histograms, counters, marks)
+    KC_ST_COUNT = 9
 };
 
 // Optional per-phase trace (profiling aid), KC_TRACE_WORDS u64 per phase:
@@ -55,6 +57,7 @@ struct PeelLaunch {
     uint32_t grid, block;
     bool wide;
     bool gpp_compat;
+    bool count_atomics; // run the variant that counts its atomics (stats)
     uint64_t timeout_ticks;
     uint64_t *trace; // may be null
     uint32_t trace_cap;

@@ -427,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     uint64_t changed_total = 0;
     uint64_t upd_edges = 0; // adjacency entries of changed vertices walked (HistoCore)
     uint64_t chg_deg = 0;   // degrees of changed vertices, summed (HistoCore)
+    uint64_t n_atom = 0;    // atomics on graph-sized arrays (histograms, marks)
     uint32_t cap_list = V ? V : 1;
     uint32_t *act_cur = a.list[0]; // CntCore active lists, swapped per round
     uint32_t *act_nxt = a.list[3];
@@ -458,16 +459,20 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         a.histo[b + j] = j == 0 ? d : L.bins[wv][j];
                 } else {
                     wave_hist<R, false>(a, v, d, a.core, a.histo + b);
-                    if (lane == 0)
+                    if (lane == 0) {
                         a.histo[b] = d;
+                        n_atom += d;
+                    }
                 }
             },
             [&](uint32_t v) {
                 uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
                 uint64_t b = hbase(a.rowptr, v);
                 block_hist<R, BLOCK>(a, v, d, a.histo + b);
-                if (threadIdx.x == 0)
+                if (threadIdx.x == 0) {
                     a.histo[b] = d;
+                    n_atom += d;
+                }
                 __syncthreads();
             });
         if (!hbar(a, gs, gen, round, 0))
@@ -601,6 +606,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         return;
                     uint32_t old = a.core[v];
                     uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old);
+                    if (lane == 0 && old >= KC_LBINS)
+                        n_atom += d; // histogram built in HBM
                     if (k == old)
                         return;
                     if (lane == 0) {
@@ -616,6 +623,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     uint32_t sum;
                     uint64_t b = hbase(a.rowptr, v);
                     block_hist<R, BLOCK>(a, v, old, a.histo + b);
+                    if (threadIdx.x == 0)
+                        n_atom += (uint64_t)(a.rowptr[v + 1] - a.rowptr[v]);
                     uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
                     bool ch = k != old;
                     if (ch && threadIdx.x == 0)
@@ -698,9 +707,12 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                                       sub == cu ? cu : KC_HEMPTY))) {
                                     uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
                                     flag = oc >= cu && c == cu;
+                                    n_atom++;
                                 }
-                                if (!(hub && hagg_add(L, (uint32_t)(bu + nc), 1, u, KC_HEMPTY)))
+                                if (!(hub && hagg_add(L, (uint32_t)(bu + nc), 1, u, KC_HEMPTY))) {
                                     atomicAdd(&a.histo[bu + nc], 1u);
+                                    n_atom++;
+                                }
                             }
                         }
                         hpush(gs, phase, f_list, cap_list, flag, u);
@@ -721,6 +733,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     int32_t d = L.agg_d[sl];
                     if (d != 0) {
                         uint32_t old = atomicAdd(&a.histo[w], (uint32_t)d);
+                        n_atom++;
                         uint32_t cu = L.agg_cu[sl];
                         if (cu != KC_HEMPTY && d < 0) {
                             u = L.agg_u[sl];
@@ -758,6 +771,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         if (i < e) {
                             u = a.col[i];
                             fresh = atomicMax(&a.mark[u], epoch) != epoch;
+                            n_atom++;
                             if (a.method == HM_CNT)
                                 a.cnt[u] = 0;
                         }
@@ -775,6 +789,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         if (i < e) {
                             u = a.col[i];
                             fresh = atomicMax(&a.mark[u], epoch) != epoch;
+                            n_atom++;
                             if (a.method == HM_CNT)
                                 a.cnt[u] = 0;
                         }
@@ -868,6 +883,9 @@ out:
     uint64_t cd = wave_sum_u64(chg_deg);
     if (lane == 0 && cd)
         atomicAdd((unsigned long long *)&a.stats[KC_ST_CHG_DEG], (unsigned long long)cd);
+    uint64_t na = wave_sum_u64(n_atom);
+    if (lane == 0 && na)
+        atomicAdd((unsigned long long *)&a.stats[KC_ST_ATOMICS], (unsigned long long)na);
 }
 
 template <typename R>

@@ -83,6 +83,7 @@ template <int BLOCK, int LQ> struct PeelLds {
     uint32_t live;   // any live vertex seen by the scan
     uint32_t rounds; // local drain rounds in the current phase (trace only)
     unsigned long long ph_edges; // edges walked in the current phase (trace only)
+    unsigned long long natom;    // atomics issued on the degree array and claim bitmap
 };
 
 template <typename R> __device__ __forceinline__ uint32_t deg0(const R *rp, uint32_t v)
@@ -145,7 +146,7 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
 // shared list whole and are claimed when the next iteration retires them.
 constexpr uint32_t KC_WHOLE = 0xffffffffu; // shared item: a whole vertex, split by its taker
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, const PeelArgs<R> &a,
                                             uint32_t phase, uint32_t level, bool claim,
                                             uint32_t u)
@@ -212,6 +213,15 @@ __device__ __forceinline__ void push_hub_chunks(const PeelArgs<R> &a, uint32_t p
 // latency of one step is paid once for WALK_U edges.
 constexpr int WALK_U = 4;
 
+// Add this thread's count of degree / claim-word atomics to the workgroup's.
+template <int BLOCK, int LQ>
+__device__ __forceinline__ void count_atomics(PeelLds<BLOCK, LQ> &L, uint32_t n)
+{
+    n = wave_sum_u32(n);
+    if (lane_id() == 0 && n)
+        atomicAdd(&L.natom, (unsigned long long)n);
+}
+
 // Per-workgroup aggregation of decrements aimed at vertices whose degree is
 // far above the level (in R-MAT these are the hubs every processed vertex
 // touches; one atomic per decrement would serialise on their addresses).
@@ -245,10 +255,11 @@ __device__ __forceinline__ bool agg_add(PeelLds<BLOCK, LQ> &L, uint32_t u)
 // first subtraction to saturate sees every earlier one already settled, so
 // the effective amounts sum to min(total, degree - level) in any
 // interleaving, as the one-at-a-time rule does.
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                           uint32_t phase, int pushbuf)
 {
+    uint32_t na = 0;
     for (uint32_t s0 = 0; s0 < KC_AGG; s0 += BLOCK) {
         uint32_t s = s0 + threadIdx.x;
         uint32_t u = L.agg_key[s];
@@ -259,6 +270,7 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
             L.agg_cnt[s] = 0;
             uint32_t old = atomicSub(&a.deg[u], c);
             uint32_t eff = old > level ? (old - level < c ? old - level : c) : 0u;
+            na += 1 + (c > eff) + (old > level && old - eff == level && a.mode == MODE_BUCKET);
             if (c > eff)
                 atomicAdd(&a.deg[u], c - eff);
             if (old - eff > level)
@@ -270,15 +282,18 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                 claim = (pv & bit) == 0;
             }
         }
-        push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, level, claim, u);
+        push_claims<BLOCK, LQ, R, CNT>(L, pushbuf, a, phase, level, claim, u);
     }
+    if constexpr (CNT)
+        count_atomics(L, na);
 }
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                            uint32_t phase, uint32_t m, int pushbuf)
 {
     const uint64_t total = L.pre[m];
+    uint32_t na = 0;
     for (uint64_t e0 = 0; e0 < total; e0 += (uint64_t)BLOCK * WALK_U) {
         uint32_t u[WALK_U];
         bool val[WALK_U];
@@ -330,6 +345,8 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
             claim[k] = false;
             if (!hit[k])
                 continue;
+            na += 1 + (a.mode != MODE_GPP && old[k] <= level) +
+                  (a.mode == MODE_BUCKET && old[k] == level + 1);
             if (old[k] > level + 1 && old[k] - 1 < nmin)
                 nmin = old[k] - 1;
             if (a.mode == MODE_GPP) {
@@ -347,14 +364,16 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         }
 #pragma unroll
         for (int k = 0; k < WALK_U; k++)
-            push_claims<BLOCK, LQ, R>(L, pushbuf, a, phase, level, claim[k], u[k]);
+            push_claims<BLOCK, LQ, R, CNT>(L, pushbuf, a, phase, level, claim[k], u[k]);
         nmin = wave_min_u32(nmin);
         if (lane_id() == 0 && nmin != 0xffffffffu)
             atomicMin(&L.minv, nmin);
     }
+    if constexpr (CNT)
+        count_atomics(L, na);
     __syncthreads();
     if (a.mode != MODE_GPP) {
-        flush_agg<BLOCK, LQ, R>(L, a, level, phase, pushbuf);
+        flush_agg<BLOCK, LQ, R, CNT>(L, a, level, phase, pushbuf);
         __syncthreads();
     }
 }
@@ -394,7 +413,7 @@ __device__ __forceinline__ void stage_prefix(PeelLds<BLOCK, LQ> &L, uint32_t m, 
 
 // Process the local frontier until it is empty.  Items claimed while
 // processing buffer b are pushed to buffer b^1.
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level, uint32_t phase,
                       uint64_t &edges)
 {
@@ -438,7 +457,7 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
                 stage_prefix<BLOCK, LQ>(L, keep, threadIdx.x < keep ? len : 0u);
             }
             edges += threadIdx.x < keep ? len : 0u;
-            walk_items<BLOCK, LQ, R>(L, a, level, phase, keep, cur ^ 1);
+            walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, keep, cur ^ 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -459,7 +478,7 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
 // drops a chunk once its claim word is full.  Every wave keeps four chunks'
 // loads in flight.  Bucket/PeelOne claims go to lbuf[0]; GPP retirees are
 // written out and listed for next phase's scatter.
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a,
                                            uint32_t level, uint32_t phase, uint32_t c,
                                            uint64_t dw, uint32_t d, bool live, uint32_t &lmin,
@@ -478,8 +497,11 @@ __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs
     if (cm) {
         uint32_t leader = __ffsll((unsigned long long)cm) - 1;
         unsigned long long prev = 0;
-        if (lane == leader)
+        if (lane == leader) {
             prev = atomicOr((unsigned long long *)&a.done[c], (unsigned long long)cm);
+            if constexpr (CNT)
+                atomicAdd(&L.natom, 1ull);
+        }
         prev = __shfl(prev, leader, KC_WAVE);
         claim = ((cm & ~(uint64_t)prev) >> lane) & 1ull;
     }
@@ -489,11 +511,11 @@ __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs
         nret += claim ? 1u : 0u;
         push_shared(a, phase, claim, (uint64_t)v);
     } else {
-        push_claims<BLOCK, LQ, R>(L, 0, a, phase, level, claim, v);
+        push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v);
     }
 }
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                            uint32_t phase)
 {
@@ -540,7 +562,7 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
                     L.chunks[i0 + NW * k] = KC_DEAD; // every vertex claimed
                 continue;
             }
-            scan_chunk<BLOCK, LQ, R>(L, a, level, phase, cc[k], dw[k], d[k], lv[k], lmin, nret);
+            scan_chunk<BLOCK, LQ, R, CNT>(L, a, level, phase, cc[k], dw[k], d[k], lv[k], lmin, nret);
         }
     }
     lmin = wave_min_u32(lmin);
@@ -575,7 +597,7 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
 // vertices of its chunks into its own region of a list (one reservation of
 // the region, then each thread writes its chunk's vertices), and the scans of
 // later levels read and compact that region instead of whole chunks.
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
 {
     const bool use_list = L.use_list != 0;
@@ -617,7 +639,7 @@ __device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
     __syncthreads();
 }
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                           uint32_t phase)
 {
@@ -625,7 +647,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
     const uint32_t base = L.lbase, n = L.lcnt;
     const uint32_t lane = lane_id();
     uint32_t *lst = a.out + base;
-    uint32_t lmin = 0xffffffffu;
+    uint32_t lmin = 0xffffffffu, ncand = 0;
     bool live_seen = false;
     if (threadIdx.x == 0)
         L.nkeep = 0;
@@ -658,13 +680,14 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                 lmin = d[k];
             live_seen |= cand || keep;
             bool claim = false;
+            ncand += cand;
             if (cand) {
                 uint64_t bit = 1ull << (v[k] & 63);
                 uint64_t pv = atomicOr((unsigned long long *)&a.done[v[k] >> 6],
                                        (unsigned long long)bit);
                 claim = (pv & bit) == 0;
             }
-            push_claims<BLOCK, LQ, R>(L, 0, a, phase, level, claim, v[k]);
+            push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v[k]);
             uint64_t km = ballot(keep);
             if (km) {
                 uint32_t leader = __ffsll((unsigned long long)km) - 1;
@@ -679,6 +702,8 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
     }
     lmin = wave_min_u32(lmin);
     uint64_t lvm = ballot(live_seen);
+    if constexpr (CNT)
+        count_atomics(L, ncand);
     if (lane == 0) {
         atomicMin(&L.minv, lmin);
         if (lvm)
@@ -696,7 +721,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
 //  kind RETIRE : items are GPP crossers, retired here, then scattered
 enum { TAKE_CHUNKS = 0, TAKE_WHOLE = 1, TAKE_RETIRE = 2 };
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                             uint32_t phase, uint32_t n_items, int kind, uint64_t &edges)
 {
@@ -734,6 +759,8 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
                 if (kind == TAKE_RETIRE) {
                     unsigned long long bit = 1ull << (v & 63);
                     unsigned long long pv = atomicOr((unsigned long long *)&a.done[v >> 6], bit);
+                    if constexpr (CNT)
+                        atomicAdd(&L.natom, 1ull);
                     if (!(pv & bit)) {
                         a.out[v] = level;
                         atomicAdd(&L.claims, 1u);
@@ -745,9 +772,9 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
         }
         edges += len;
         stage_prefix<BLOCK, LQ>(L, m, len);
-        walk_items<BLOCK, LQ, R>(L, a, level, phase, m, 0);
+        walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, m, 0);
         if (a.mode != MODE_GPP)
-            drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+            drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
     }
 }
 
@@ -766,7 +793,7 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 // and every reserved slot at or past the tail is dropped.  Ready items are always processed
 // before waiting on the rest of a reservation, so no workgroup sits on work
 // another one waits for.
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                             uint32_t phase, uint64_t &edges)
 {
@@ -886,8 +913,8 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
                 len = threadIdx.x < k ? L.hlen[threadIdx.x] : 0u;
                 edges += len;
                 stage_prefix<BLOCK, LQ>(L, k, len);
-                walk_items<BLOCK, LQ, R>(L, a, level, phase, k, 0);
-                drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+                walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, k, 0);
+                drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
                 vm_drain(); // the pushes these items caused come first
                 __syncthreads();
                 if (threadIdx.x == 0)
@@ -940,7 +967,7 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 // phase kinds of the persistent loop
 enum { PH_SCAN = 0, PH_SHARED = 1, PH_GPP_SCATTER = 2, PH_GPP_CROSS = 3 };
 
-template <int BLOCK, int LQ, typename R>
+template <int BLOCK, int LQ, typename R, bool CNT>
 __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
 {
     __shared__ PeelLds<BLOCK, LQ> L;
@@ -973,6 +1000,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     if (threadIdx.x == 0) {
         L.cnt[0] = 0;
         L.cnt[1] = 0;
+        L.natom = 0;
         L.pend[0] = 0;
         L.pend[1] = 0;
         L.lmode = 0;
@@ -1006,23 +1034,23 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         if (kind == PH_SCAN) {
             if (a.mode == MODE_BUCKET && !L.lmode && a.live_thresh &&
                 (uint64_t)a.V - total <= a.live_thresh)
-                build_live_list<BLOCK, LQ, R>(L, a);
+                build_live_list<BLOCK, LQ, R, CNT>(L, a);
             if (L.lmode)
-                scan_list<BLOCK, LQ, R>(L, a, level, phase);
+                scan_list<BLOCK, LQ, R, CNT>(L, a, level, phase);
             else
-                scan_slice<BLOCK, LQ, R>(L, a, level, phase);
+                scan_slice<BLOCK, LQ, R, CNT>(L, a, level, phase);
             if (tr && threadIdx.x == 0)
                 atomicMax((unsigned long long *)&tr[12], (unsigned long long)now_ticks());
             if (a.mode != MODE_GPP) {
-                drain<BLOCK, LQ, R>(L, a, level, phase, edges);
+                drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
                 if (tr && threadIdx.x == 0)
                     atomicMax((unsigned long long *)&tr[13], (unsigned long long)now_ticks());
-                help_shared<BLOCK, LQ, R>(L, a, level, phase, edges);
+                help_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
             }
         } else if (kind == PH_SHARED) {
-            take_shared<BLOCK, LQ, R>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
+            take_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
         } else {
-            take_shared<BLOCK, LQ, R>(L, a, level, phase, n_prev,
+            take_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, n_prev,
                                       kind == PH_GPP_CROSS ? TAKE_RETIRE : TAKE_WHOLE, edges);
         }
         __syncthreads();
@@ -1141,6 +1169,11 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     uint64_t ew = wave_sum_u64(edges);
     if (lane_id() == 0 && ew)
         atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], (unsigned long long)ew);
+    if constexpr (CNT) {
+        __syncthreads();
+        if (threadIdx.x == 0 && L.natom)
+            atomicAdd((unsigned long long *)&a.stats[KC_ST_ATOMICS], L.natom);
+    }
 }
 
 template <typename R>
@@ -1223,8 +1256,9 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     if (e != hipSuccess)
         return e;
     void *args[] = {&a};
-    e = launch_persistent((const void *)peel_kernel<BLOCK, LQ, R>, dim3(p.grid), dim3(BLOCK),
-                          args, st);
+    const void *fn = p.count_atomics ? (const void *)peel_kernel<BLOCK, LQ, R, true>
+                                     : (const void *)peel_kernel<BLOCK, LQ, R, false>;
+    e = launch_persistent(fn, dim3(p.grid), dim3(BLOCK), args, st);
     if (e != hipSuccess)
         return e;
     if (p.mode == MODE_GPP && !p.gpp_compat && p.V)
@@ -1233,17 +1267,20 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     return hipGetLastError();
 }
 
-template <int BLOCK, int LQ> static int occupancy_t(bool wide)
+template <int BLOCK, int LQ, typename R, bool CNT> static int occupancy_k()
 {
     int nb = 0;
-    hipError_t e;
-    if (wide)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, peel_kernel<BLOCK, LQ, uint64_t>,
-                                                         BLOCK, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, peel_kernel<BLOCK, LQ, uint32_t>,
-                                                         BLOCK, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, peel_kernel<BLOCK, LQ, R, CNT>, BLOCK, 0);
     return e == hipSuccess ? nb : 0;
+}
+
+// The grid must be co-resident with either variant (counting or not).
+template <int BLOCK, int LQ> static int occupancy_t(bool wide)
+{
+    int a = wide ? occupancy_k<BLOCK, LQ, uint64_t, false>() : occupancy_k<BLOCK, LQ, uint32_t, false>();
+    int b = wide ? occupancy_k<BLOCK, LQ, uint64_t, true>() : occupancy_k<BLOCK, LQ, uint32_t, true>();
+    return a < b ? a : b;
 }
 
 int peel_occupancy(uint32_t block, bool wide)

@@ -48,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
                 ("prep_ms", ctypes.c_double), ("changed_degree", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint64 * 1)]
+                ("atomics", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_
@@ -787,6 +787,7 @@ def _trace_sig():
 
 KC_OPT_TRACE = 1
 KC_OPT_NO_DEGSORT = 2
+KC_OPT_COUNT = 4
 
 
 def device_trace(dgraph, max_phases=1 << 16):

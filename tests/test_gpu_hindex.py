@@ -111,3 +111,19 @@ def test_histocore_adjacency_order_does_not_matter(ctx, flags):
     # the final histogram are the same with and without it
     hg = pa.HostGraph.chunglu(1 << 14, beta=2.1, mean_deg=20.0, cap=6000.0, seed=8)
     _check(ctx, hg.rowptr, hg.col, "histocore", hist=True, flags=flags)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_atomic_count_bounds(ctx, method):
+    hg = pa.HostGraph.rmat(13, 16, seed=5)
+    dg = ctx.upload(hg)
+    try:
+        _, st = dg.decompose(method)
+    finally:
+        dg.close()
+    assert st["changed"] > 0 and st["atomics"] > 0
+    if method == "histocore":
+        # initial histograms: at most one atomic per adjacency entry; each
+        # update-walk entry at most a bin decrement and a bin increment, or
+        # their summed forms
+        assert st["atomics"] <= hg.nnz + 2 * st["edges_walked"]

@@ -145,3 +145,20 @@ def test_repeat_is_deterministic(ctx):
             first = core
         np.testing.assert_array_equal(core, first)
     dg.close()
+
+
+@pytest.mark.parametrize("method", PEELS)
+def test_atomic_count_bounds(ctx, method):
+    # every walked adjacency entry issues at most a decrement plus a restore
+    # or a claim; every vertex at most one scan claim
+    hg = pa.HostGraph.rmat(13, 16, seed=5)
+    dg = ctx.upload(hg)
+    try:
+        core, st = dg.decompose(method, flags=pa.kcore.KC_OPT_COUNT)
+        _, plain = dg.decompose(method)
+    finally:
+        dg.close()
+    np.testing.assert_array_equal(core, oracle.bz(hg.rowptr, hg.col))
+    assert st["edges_walked"] > 0
+    assert 0 < st["atomics"] <= 3 * st["edges_walked"] + hg.V
+    assert plain["atomics"] == 0  # the default variant does not count
