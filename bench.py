@@ -438,6 +438,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": peel_traffic,
                      "kernel": "peel_kernel", "kernel_ms": round(kernel_ms, 4),
+                     "launches_per_step": st["stages"],
                      "kernel_ms_median": round(float(np.median(dev_ms)), 4),
                      "kernel_ms_trimmed": round(trimmed_mean(dev_ms), 4),
                      "algorithmic_bytes": b_alg, "atomics": cst["atomics"],

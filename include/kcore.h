@@ -83,7 +83,11 @@ typedef struct kc_opts {
     uint32_t agg_div;    /* bucket peels: read each target's degree before decrementing and
                             sum hub decrements in LDS once at most V/agg_div vertices are
                             live (0: 64; 0xffffffff: from the start) */
-    uint32_t reserved[3];
+    uint32_t compact_div; /* bucket peels: once at most V/compact_div vertices are live (and
+                             at least 4,096), continue on a compacted copy of the live
+                             subgraph, again at each further 1/compact_div; 0: default (8),
+                             0xffffffff: never */
+    uint32_t reserved[2];
 } kc_opts;
 
 typedef struct kc_stats {
@@ -96,7 +100,8 @@ typedef struct kc_stats {
     uint32_t kmax;
     uint32_t grid;
     uint32_t block;
-    uint32_t reserved0;
+    uint32_t stages;  /* peel kernel launches: 1, plus one per compaction of the live
+                         subgraph (bucket peels, kc_opts.compact_div) */
     double device_ms; /* HIP-event time of the decomposition on its stream */
     double total_ms;  /* host wall time of the call incl. copies */
     uint64_t changed; /* h-index: core-value changes over all rounds */

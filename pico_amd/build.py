@@ -28,6 +28,7 @@ HIP_FLAGS = [
 
 LIB_SOURCES = [
     os.path.join(CSRC, "kc_peel.hip"),
+    os.path.join(CSRC, "kc_compact.hip"),
     os.path.join(CSRC, "kc_hindex.hip"),
     os.path.join(CSRC, "kc_prep.hip"),
     os.path.join(CSRC, "kc_batch.hip"),

@@ -892,7 +892,7 @@ kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *bes
     }
     kc_opts bo{};
     double bms = 1e300;
-    for (const kc_opts &o : cand) {
+    auto trial = [&](const kc_opts &o) -> kc_status {
         double t = 1e300;
         for (uint32_t r = 0; r < reps; r++) {
             kc_stats st{};
@@ -904,6 +904,20 @@ kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *bes
         if (t < bms) {
             bms = t;
             bo = o;
+        }
+        return KC_OK;
+    };
+    for (const kc_opts &o : cand)
+        if (kc_status s = trial(o))
+            return s;
+    // the bucket peels' compaction ratio, with the best of the rest
+    if (method == KC_PEEL_DYN || method == KC_PEELONE_DYN) {
+        const kc_opts base = bo;
+        for (uint32_t cd : {4u, 16u, 0xffffffffu}) {
+            kc_opts o = base;
+            o.compact_div = cd;
+            if (kc_status s = trial(o))
+                return s;
         }
     }
     g->tuned[method] = bo;

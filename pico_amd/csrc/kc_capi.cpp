@@ -392,7 +392,10 @@ void kc_graph_free(kc_graph *g)
     }
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
-                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds})
+                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->cs_row[0], &g->cs_row[1],
+                      &g->cs_col[0], &g->cs_col[1], &g->cs_deg[0], &g->cs_deg[1], &g->cs_done[0],
+                      &g->cs_done[1], &g->cs_gmap[0], &g->cs_gmap[1], &g->cs_loc, &g->cs_wpre,
+                      &g->cs_tmp, &g->cs_bad})
         if (b->p)
             hipFree(b->p);
     if (g->ev0)
@@ -461,7 +464,7 @@ __global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
 static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     kc_ctx *c = g->ctx;
-    uint32_t block = o.block ? o.block : 1024;
+    uint32_t block = o.block ? o.block : 512;
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
     int occ = kc::peel_occupancy(block, g->wide);
@@ -529,11 +532,125 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         p.trace = (uint64_t *)g->trace.p;
         p.trace_cap = KC_TRACE_CAP;
     }
+    // Bucket peels continue on compacted live subgraphs once few vertices
+    // remain (kc_compact.hip); stage 0 is the graph itself.
+    const uint32_t cdiv = o.compact_div ? o.compact_div : 8u;
+    auto tail_of = [&](uint32_t Vs) -> uint32_t {
+        if (mode != 0 || cdiv == 0xffffffffu || cdiv < 2)
+            return 0u;
+        uint32_t t = Vs / cdiv;
+        return t >= 4096u ? t : 0u;
+    };
+    p.tail_live = tail_of(g->V);
+    p.start_level = 0;
     std::unique_lock<std::mutex> coop(coop_mutex(c->device));
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_peel(p, c->stream);
     if (e != hipSuccess)
         return hip_fail(e, "peel launch");
+    uint64_t sv[kc::KC_ST_COUNT];
+    uint64_t acc_levels = 0, acc_phases = 0, acc_edges = 0, acc_atomics = 0;
+    bool compacted = false;
+    uint32_t nstages = 1;
+    for (int stage = 0;; stage++) {
+        KC_HIP(hipMemcpyAsync(sv, g->stats.p, sizeof sv, hipMemcpyDeviceToHost, c->stream),
+               "stats copy");
+        KC_HIP(hipStreamSynchronize(c->stream), "peel stage");
+        if (sv[kc::KC_ST_ERR] == 1)
+            return fail(KC_ETIMEOUT, "device wait timed out");
+        if (sv[kc::KC_ST_ERR] == 2)
+            return fail(KC_EOVERFLOW, "shared work list overflow (capacity %u items)",
+                        p.list_cap);
+        if (sv[kc::KC_ST_ERR])
+            return fail(KC_EDEVICE, "device consistency check failed (%llu)",
+                        (unsigned long long)sv[kc::KC_ST_ERR]);
+        acc_levels += sv[kc::KC_ST_LEVELS_RUN];
+        acc_phases += sv[kc::KC_ST_PHASES];
+        acc_edges += sv[kc::KC_ST_EDGES];
+        acc_atomics += sv[kc::KC_ST_ATOMICS];
+        const int cur = stage & 1, nxt = (stage + 1) & 1;
+        if (stage > 0)
+            KC_HIP(kc::scatter_cores(p.deg, (const uint32_t *)g->cs_gmap[cur].p, p.V,
+                                     (uint32_t *)g->deg.p, c->stream),
+                   "core scatter");
+        if (p.tail_live == 0 || sv[kc::KC_ST_TAIL_LEVEL] == 0xffffffffull)
+            break;
+        // compact the live subgraph of this stage into the other stage set
+        const uint32_t Vn = (uint32_t)sv[kc::KC_ST_LIVE];
+        const uint32_t nw = (p.V + 63) / 64;
+        const bool wide_n = g->nnz >= (1ull << 32);
+        const size_t rb = wide_n ? 8 : 4;
+        const size_t tb = kc::compact_tmp_bytes(p.V, Vn);
+        if ((s = ensure(g, g->cs_wpre, (size_t)(nw + 1) * 4, "compaction")) ||
+            (s = ensure(g, g->cs_loc, (size_t)Vn * 4 + 4, "compaction")) ||
+            (s = ensure(g, g->cs_gmap[nxt], (size_t)Vn * 4 + 4, "compaction")) ||
+            (s = ensure(g, g->cs_row[nxt], (size_t)(Vn + 1) * rb, "compaction")) ||
+            (s = ensure(g, g->cs_tmp, tb, "compaction")) ||
+            (s = ensure(g, g->cs_bad, 4, "compaction")))
+            return s;
+        if (!compacted)
+            KC_HIP(hipMemsetAsync(g->cs_bad.p, 0, 4, c->stream), "memset");
+        compacted = true;
+        kc::CompactIn in{p.rowptr, p.wide, p.col, p.V, p.deg, p.done,
+                         stage > 0 ? (const uint32_t *)g->cs_gmap[cur].p : nullptr};
+        kc::CompactOut co{};
+        co.Vn = Vn;
+        co.wpre = (uint32_t *)g->cs_wpre.p;
+        co.loc = (uint32_t *)g->cs_loc.p;
+        co.gmap = (uint32_t *)g->cs_gmap[nxt].p;
+        co.rowptr = g->cs_row[nxt].p;
+        co.wide = wide_n;
+        co.bad = (uint32_t *)g->cs_bad.p;
+        co.tmp = g->cs_tmp.p;
+        co.tmp_bytes = g->cs_tmp.n;
+        KC_HIP(kc::compact_rows(in, co, c->stream), "compaction rows");
+        uint32_t live_chk = 0, bad = 0;
+        uint64_t nnz_n = 0;
+        KC_HIP(hipMemcpyAsync(&live_chk, co.wpre + nw, 4, hipMemcpyDeviceToHost, c->stream),
+               "compaction copy");
+        KC_HIP(hipMemcpyAsync(&nnz_n, (char *)co.rowptr + (size_t)Vn * rb, rb,
+                              hipMemcpyDeviceToHost, c->stream),
+               "compaction copy");
+        KC_HIP(hipMemcpyAsync(&bad, co.bad, 4, hipMemcpyDeviceToHost, c->stream),
+               "compaction copy");
+        KC_HIP(hipStreamSynchronize(c->stream), "compaction rows");
+        if (live_chk != Vn || bad)
+            return fail(KC_EDEVICE, "compaction: %u live vertices counted, %u expected (flags %u)",
+                        live_chk, Vn, bad);
+        if ((s = ensure(g, g->cs_col[nxt], (size_t)nnz_n * 4 + 4, "compaction")) ||
+            (s = ensure(g, g->cs_deg[nxt], (size_t)Vn * 4 + 4, "compaction")) ||
+            (s = ensure(g, g->cs_done[nxt], (size_t)((Vn + 63) / 64) * 8 + 8, "compaction")))
+            return s;
+        co.col = (uint32_t *)g->cs_col[nxt].p;
+        KC_HIP(kc::compact_cols(in, co, c->stream), "compaction adjacency");
+        // every row must be complete before the next stage reads it
+        KC_HIP(hipMemcpyAsync(&bad, co.bad, 4, hipMemcpyDeviceToHost, c->stream),
+               "compaction copy");
+        KC_HIP(hipStreamSynchronize(c->stream), "compaction adjacency");
+        if (bad)
+            return fail(KC_EDEVICE, "compaction: row lengths differ from degrees (flags %u)", bad);
+        // the next stage: same launch shape, the level where this one stopped
+        p.rowptr = co.rowptr;
+        p.wide = wide_n;
+        p.col = co.col;
+        p.V = Vn;
+        p.deg = (uint32_t *)g->cs_deg[nxt].p;
+        p.done = (uint64_t *)g->cs_done[nxt].p;
+        uint64_t cap_n = (uint64_t)Vn + nnz_n / hub + 2ull * grid + 64;
+        p.list_cap = (uint32_t)std::min<uint64_t>(cap_n, cap);
+        p.live_thresh = Vn / 4;
+        p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : Vn / agg_div;
+        p.tail_live = tail_of(Vn);
+        p.start_level = (uint32_t)sv[kc::KC_ST_TAIL_LEVEL];
+        if (p.trace) {
+            uint64_t used = std::min<uint64_t>(acc_phases, KC_TRACE_CAP);
+            p.trace = (uint64_t *)g->trace.p + used * kc::KC_TRACE_WORDS;
+            p.trace_cap = (uint32_t)(KC_TRACE_CAP - used);
+        }
+        if ((e = kc::launch_peel(p, c->stream)) != hipSuccess)
+            return hip_fail(e, "peel launch");
+        nstages++;
+    }
     KC_HIP(hipEventRecord(g->ev1, c->stream), "hipEventRecord");
     KC_HIP(hipMemsetAsync(g->kmax.p, 0, 4, c->stream), "memset");
     g->result = mode == 2 ? (const uint32_t *)g->out.p : (const uint32_t *)g->deg.p;
@@ -543,27 +660,19 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     KC_HIP(hipStreamSynchronize(c->stream), "decomposition");
     float ms = 0;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
-    uint64_t sv[kc::KC_ST_COUNT];
     uint32_t km = 0;
-    KC_HIP(hipMemcpy(sv, g->stats.p, sizeof sv, hipMemcpyDeviceToHost), "stats copy");
     KC_HIP(hipMemcpy(&km, g->kmax.p, 4, hipMemcpyDeviceToHost), "kmax copy");
-    if (sv[kc::KC_ST_ERR] == 1)
-        return fail(KC_ETIMEOUT, "device wait timed out");
-    if (sv[kc::KC_ST_ERR] == 2)
-        return fail(KC_EOVERFLOW, "shared work list overflow (capacity %u items)", cap);
-    if (sv[kc::KC_ST_ERR])
-        return fail(KC_EDEVICE, "device consistency check failed (%llu)",
-                    (unsigned long long)sv[kc::KC_ST_ERR]);
     st.device_ms = ms;
     st.kmax = km;
     st.levels = g->V ? (uint64_t)km + 1 : 0;
-    st.levels_run = sv[kc::KC_ST_LEVELS_RUN];
-    st.phases = sv[kc::KC_ST_PHASES];
-    st.edges_walked = sv[kc::KC_ST_EDGES];
-    st.atomics = sv[kc::KC_ST_ATOMICS];
+    st.levels_run = acc_levels;
+    st.phases = acc_phases;
+    st.edges_walked = acc_edges;
+    st.atomics = acc_atomics;
+    st.stages = nstages;
     st.grid = grid;
     st.block = block;
-    if (p.trace)
+    if (o.flags & KC_OPT_TRACE)
         g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     if (method == KC_PEEL_DYN || method == KC_PEELONE_DYN)
         st.iterations = st.levels; // peel-dyn.cc:155-173: one trip per level
@@ -702,6 +811,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     st.grid = grid;
     st.block = block;
+    st.stages = 1;
     return KC_OK;
 }
 

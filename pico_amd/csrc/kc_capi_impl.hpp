@@ -55,6 +55,8 @@ struct kc_graph {
     // method scratch, allocated on first use and kept for reuse
     DevBuf deg, out, done, list0, list1, gs, stats, histo, cnt, flags, hl0, hl1, hl2, kmax, trace;
     DevBuf col_ds, nd_ds, hb_ds; // degree-ordered adjacency (built on first HistoCore run)
+    // bucket peel on compacted live subgraphs: two alternating stage sets
+    DevBuf cs_row[2], cs_col[2], cs_deg[2], cs_done[2], cs_gmap[2], cs_loc, cs_wpre, cs_tmp, cs_bad;
     bool degsorted = false;
     uint32_t trace_phases = 0;
     const uint32_t *result = nullptr;
@@ -78,6 +80,8 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
         o.local_cap = t.local_cap;
     if (!o.agg_div)
         o.agg_div = t.agg_div;
+    if (!o.compact_div)
+        o.compact_div = t.compact_div;
     return o;
 }
 

@@ -1,0 +1,267 @@
+// kc_compact.hip -- residual compaction for the bucket peel.
+//
+// Once few vertices are live, a level of the peel still walks whole rows
+// (edges to claimed vertices included) over a degree array spread across the
+// full vertex range.  The peel kernel can stop at a level start; the live
+// vertices are then renumbered in id order (a prefix over the claim bitmap)
+// and their live-live adjacency copied into a compact CSR, on which the peel
+// continues from the same level.  The k-cores for k >= that level are
+// subgraphs of the live subgraph, so the core numbers it yields are those of
+// the whole graph.
+//
+// Rows are copied in order: short rows by one wave, long rows by a whole
+// workgroup with a block-wide prefix per step, so the result is deterministic.
+// The live count of each row must equal the vertex's residual degree (every
+// claimed neighbour decremented it exactly once); a mismatch is reported, and
+// no write ever goes past the row's end.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "kc_device.hpp"
+#include "kc_engines.hpp"
+
+namespace kc {
+
+namespace {
+
+constexpr uint32_t CB = 256;          // threads per block in these kernels
+constexpr uint32_t LONG_ROW = 1024;   // rows longer than this go to a workgroup
+
+__global__ void live_word_count(const uint64_t *done, uint32_t nw, uint32_t *cnt)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw)
+        cnt[i] = (uint32_t)__popcll(~done[i]);
+    else if (i == nw)
+        cnt[nw] = 0;
+}
+
+template <typename R>
+__global__ void renumber(const R *rowptr, const uint64_t *done, const uint32_t *wpre,
+                         const uint32_t *deg, const uint32_t *gmap_in, uint32_t V, uint32_t Vn,
+                         uint32_t *loc, uint32_t *gmap_out, uint32_t *rowlen, uint32_t *bad)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0)
+        rowlen[Vn] = 0;
+    if (v >= V)
+        return;
+    uint64_t freew = ~done[v >> 6];
+    uint32_t b = v & 63;
+    if (!((freew >> b) & 1ull))
+        return;
+    uint32_t i = wpre[v >> 6] + (uint32_t)__popcll(freew & ((1ull << b) - 1ull));
+    if (i >= Vn) { // live count differs from the peel's
+        atomicOr(bad, 1u);
+        return;
+    }
+    loc[i] = v;
+    gmap_out[i] = gmap_in ? gmap_in[v] : v;
+    uint32_t d = deg[v];
+    uint64_t full = (uint64_t)(rowptr[v + 1] - rowptr[v]);
+    if (d > full) {
+        atomicOr(bad, 2u);
+        d = 0;
+    }
+    rowlen[i] = d;
+}
+
+// New id of u if live, else ~0.
+__device__ __forceinline__ uint32_t new_id(const uint64_t *done, const uint32_t *wpre, uint32_t u)
+{
+    uint64_t freew = ~done[u >> 6];
+    uint32_t b = u & 63;
+    if (!((freew >> b) & 1ull))
+        return 0xffffffffu;
+    return wpre[u >> 6] + (uint32_t)__popcll(freew & ((1ull << b) - 1ull));
+}
+
+// One wave per short row.
+template <typename R, typename R2>
+__global__ __launch_bounds__(CB) void copy_short(const R *rowptr, const uint32_t *col,
+                                                 const uint64_t *done, const uint32_t *wpre,
+                                                 const uint32_t *loc, uint32_t Vn,
+                                                 const R2 *orow, uint32_t *ocol, uint32_t *bad)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (CB / KC_WAVE);
+    for (uint32_t i = blockIdx.x * (CB / KC_WAVE) + threadIdx.x / KC_WAVE; i < Vn; i += nwaves) {
+        uint32_t v = loc[i];
+        uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
+        if (e - s > LONG_ROW)
+            continue;
+        uint64_t ob = (uint64_t)orow[i], oe = (uint64_t)orow[i + 1];
+        uint64_t pos = ob;
+        for (uint64_t j0 = s; j0 < e; j0 += KC_WAVE) {
+            uint64_t j = j0 + lane;
+            uint32_t id = j < e ? new_id(done, wpre, col[j]) : 0xffffffffu;
+            uint64_t m = ballot(id != 0xffffffffu);
+            uint64_t p = pos + lane_rank(m);
+            if (id != 0xffffffffu && p < oe)
+                ocol[p] = id;
+            pos += (uint32_t)__popcll(m);
+        }
+        if (lane == 0 && pos != oe)
+            atomicOr(bad, 4u);
+    }
+}
+
+// One workgroup per long row, 4 entries per thread per step.
+template <typename R, typename R2>
+__global__ __launch_bounds__(CB) void copy_long(const R *rowptr, const uint32_t *col,
+                                                const uint64_t *done, const uint32_t *wpre,
+                                                const uint32_t *loc, uint32_t Vn, const R2 *orow,
+                                                uint32_t *ocol, uint32_t *bad)
+{
+    constexpr uint32_t U = 4;
+    constexpr uint32_t NW = CB / KC_WAVE;
+    __shared__ uint32_t s_w[NW * U];
+    const uint32_t lane = lane_id(), w = threadIdx.x / KC_WAVE;
+    for (uint32_t i = blockIdx.x; i < Vn; i += gridDim.x) {
+        uint32_t v = loc[i];
+        uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
+        if (e - s <= LONG_ROW)
+            continue;
+        uint64_t oe = (uint64_t)orow[i + 1];
+        uint64_t pos = (uint64_t)orow[i];
+        for (uint64_t j0 = s; j0 < e; j0 += (uint64_t)CB * U) {
+            uint32_t id[U];
+            uint64_t m[U];
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++) {
+                uint64_t j = j0 + (uint64_t)k * CB + threadIdx.x;
+                id[k] = j < e ? new_id(done, wpre, col[j]) : 0xffffffffu;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++) {
+                m[k] = ballot(id[k] != 0xffffffffu);
+                if (lane == 0)
+                    s_w[k * NW + w] = (uint32_t)__popcll(m[k]);
+            }
+            __syncthreads();
+            // entries are ordered by (k, wave, lane): prefix over s_w in that order
+            uint32_t before = 0, total = 0;
+            for (uint32_t t = 0; t < NW * U; t++) {
+                uint32_t c = s_w[t];
+                total += c;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++) {
+                before = 0;
+                for (uint32_t t = 0; t < k * NW + w; t++)
+                    before += s_w[t];
+                uint64_t p = pos + before + lane_rank(m[k]);
+                if (id[k] != 0xffffffffu && p < oe)
+                    ocol[p] = id[k];
+            }
+            pos += total;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && pos != oe)
+            atomicOr(bad, 8u);
+    }
+}
+
+__global__ void scatter_kernel(const uint32_t *sub, const uint32_t *gmap, uint32_t n,
+                               uint32_t *core)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        core[gmap[i]] = sub[i];
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename R2> size_t scan_bytes(uint32_t n)
+{
+    size_t b = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const uint32_t *)nullptr, (R2 *)nullptr, n);
+    return b;
+}
+
+} // namespace
+
+size_t compact_tmp_bytes(uint32_t V, uint32_t Vn)
+{
+    uint32_t nw = (V + 63) / 64;
+    size_t a = scan_bytes<uint32_t>(nw + 1);
+    size_t b = scan_bytes<uint64_t>(Vn + 1);
+    size_t c = scan_bytes<uint32_t>(Vn + 1);
+    return align256((size_t)(nw + 1) * 4) + align256((size_t)(Vn + 1) * 4) +
+           align256(std::max(a, std::max(b, c)));
+}
+
+hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st)
+{
+    uint32_t nw = (in.V + 63) / 64;
+    char *t = (char *)out.tmp;
+    uint32_t *cnt = (uint32_t *)t;
+    uint32_t *rowlen = (uint32_t *)(t + align256((size_t)(nw + 1) * 4));
+    void *ctmp = t + align256((size_t)(nw + 1) * 4) + align256((size_t)(out.Vn + 1) * 4);
+    size_t cbytes = out.tmp_bytes - (size_t)((char *)ctmp - t);
+    hipLaunchKernelGGL(live_word_count, dim3((nw + 1 + CB - 1) / CB), dim3(CB), 0, st, in.done,
+                       nw, cnt);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(ctmp, cbytes, cnt, out.wpre, nw + 1, st);
+    if (e != hipSuccess)
+        return e;
+    if (in.V) {
+        if (in.wide)
+            hipLaunchKernelGGL(renumber<uint64_t>, dim3((in.V + CB - 1) / CB), dim3(CB), 0, st,
+                               (const uint64_t *)in.rowptr, in.done, out.wpre, in.deg, in.gmap,
+                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad);
+        else
+            hipLaunchKernelGGL(renumber<uint32_t>, dim3((in.V + CB - 1) / CB), dim3(CB), 0, st,
+                               (const uint32_t *)in.rowptr, in.done, out.wpre, in.deg, in.gmap,
+                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad);
+    }
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    if (out.wide)
+        return hipcub::DeviceScan::ExclusiveSum(ctmp, cbytes, rowlen, (uint64_t *)out.rowptr,
+                                                out.Vn + 1, st);
+    return hipcub::DeviceScan::ExclusiveSum(ctmp, cbytes, rowlen, (uint32_t *)out.rowptr,
+                                            out.Vn + 1, st);
+}
+
+template <typename R, typename R2>
+static void launch_copy(const CompactIn &in, const CompactOut &out, hipStream_t st)
+{
+    uint32_t gs = std::min<uint32_t>(4096, (out.Vn + 3) / 4 + 1);
+    hipLaunchKernelGGL((copy_short<R, R2>), dim3(gs), dim3(CB), 0, st, (const R *)in.rowptr,
+                       in.col, in.done, out.wpre, out.loc, out.Vn, (const R2 *)out.rowptr,
+                       out.col, out.bad);
+    uint32_t gl = std::min<uint32_t>(2048, out.Vn ? out.Vn : 1);
+    hipLaunchKernelGGL((copy_long<R, R2>), dim3(gl), dim3(CB), 0, st, (const R *)in.rowptr, in.col,
+                       in.done, out.wpre, out.loc, out.Vn, (const R2 *)out.rowptr, out.col,
+                       out.bad);
+}
+
+hipError_t compact_cols(const CompactIn &in, const CompactOut &out, hipStream_t st)
+{
+    if (out.Vn == 0)
+        return hipSuccess;
+    if (in.wide) {
+        if (out.wide)
+            launch_copy<uint64_t, uint64_t>(in, out, st);
+        else
+            launch_copy<uint64_t, uint32_t>(in, out, st);
+    } else {
+        if (out.wide)
+            launch_copy<uint32_t, uint64_t>(in, out, st);
+        else
+            launch_copy<uint32_t, uint32_t>(in, out, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t scatter_cores(const uint32_t *sub, const uint32_t *gmap, uint32_t n, uint32_t *core,
+                         hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(scatter_kernel, dim3((n + CB - 1) / CB), dim3(CB), 0, st, sub, gmap, n,
+                       core);
+    return hipGetLastError();
+}
+
+} // namespace kc

@@ -23,7 +23,9 @@ enum {
     KC_ST_CHG_DEG = 7,    // h-index: degrees of changed vertices, summed
     KC_ST_ATOMICS = 8,    // atomics issued on graph-sized arrays (degrees, claim
                           // bitmap, histograms, counters, marks)
-    KC_ST_COUNT = 9
+    KC_ST_TAIL_LEVEL = 9, // peel: level at which the kernel stopped early, or ~0
+    KC_ST_LIVE = 10,      // peel: vertices still live when the kernel ended
+    KC_ST_COUNT = 11
 };
 
 // Optional per-phase trace (profiling aid), KC_TRACE_WORDS u64 per phase:
@@ -52,6 +54,8 @@ struct PeelLaunch {
     uint32_t *out; // GPP final core; bucket: the live-vertex list (V entries)
     uint32_t live_thresh; // bucket: scan a live-vertex list once <= this many remain
     uint32_t agg_live;    // pre-check degrees and aggregate hub decrements once <= this many remain
+    uint32_t tail_live;   // bucket: stop at a level start once <= this many remain (0: never)
+    uint32_t start_level; // bucket: first level to visit
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;
@@ -64,6 +68,39 @@ struct PeelLaunch {
 };
 
 hipError_t launch_peel(const PeelLaunch &p, hipStream_t st);
+
+// Residual compaction (kc_compact.hip).  The live vertices of a stopped peel
+// (claim bit clear) are renumbered in id order and their live-live adjacency
+// copied, in row order, into a new CSR.
+struct CompactIn {
+    const void *rowptr; // the stage graph (u32 or u64 rows)
+    bool wide;
+    const uint32_t *col;
+    uint32_t V;
+    const uint32_t *deg;   // residual degrees of the live vertices
+    const uint64_t *done;  // claim bitmap
+    const uint32_t *gmap;  // stage id -> original id, or null for the original graph
+};
+struct CompactOut {
+    uint32_t Vn;          // live vertices (known from the peel's stats)
+    uint32_t *wpre;       // (V+63)/64 + 1 words: live vertices before each claim word
+    uint32_t *loc;        // Vn: stage id of each new vertex
+    uint32_t *gmap;       // Vn: original id of each new vertex
+    void *rowptr;         // Vn+1, u32 (wide == false) or u64
+    bool wide;
+    uint32_t *col;        // rowptr[Vn] entries
+    uint32_t *bad;        // set non-zero if a row's live count differs from its degree
+    void *tmp;            // scan scratch
+    size_t tmp_bytes;
+};
+// Phase 1: renumbering and row starts; *nnz_dev receives rowptr[Vn].
+hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st);
+size_t compact_tmp_bytes(uint32_t V, uint32_t Vn);
+// Phase 2: the adjacency.
+hipError_t compact_cols(const CompactIn &in, const CompactOut &out, hipStream_t st);
+// core[gmap[i]] = sub[i] for i < n
+hipError_t scatter_cores(const uint32_t *sub, const uint32_t *gmap, uint32_t n, uint32_t *core,
+                         hipStream_t st);
 int peel_occupancy(uint32_t block, bool wide);
 
 struct HindexLaunch {

@@ -41,6 +41,8 @@ template <typename R> struct PeelArgs {
     uint32_t *out; // GPP: final core; bucket: live-vertex list
     uint32_t live_thresh;
     uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
+    uint32_t tail_live;   // bucket: stop at a level start once <= this many are live
+    uint32_t start_level; // bucket: first level (every vertex's degree is >= it)
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional: KC_TRACE_WORDS per phase, see kc_engines.hpp
@@ -987,7 +989,8 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     }
 
     uint32_t gen = 0;
-    uint32_t level = (a.mode == MODE_GPP) ? 1u : 0u;
+    uint32_t level = (a.mode == MODE_GPP) ? 1u : (a.mode == MODE_BUCKET ? a.start_level : 0u);
+    uint32_t tail_level = 0xffffffffu; // level at which the kernel stopped early
     uint64_t total = 0; // vertices claimed so far (identical in all WGs)
     uint64_t lvl_claims = 0;
     uint64_t edges = 0;
@@ -1012,6 +1015,14 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         L.agg_cnt[s] = 0;
     }
     for (;;) {
+        // Few vertices left: stop before this level's scan so the caller can
+        // compact the live subgraph and continue on it (the state is that of
+        // a completed level; every workgroup takes the same decision).
+        if (a.mode == MODE_BUCKET && kind == PH_SCAN && a.tail_live && total < a.V &&
+            (uint64_t)a.V - total <= a.tail_live) {
+            tail_level = level;
+            break;
+        }
         if (threadIdx.x == 0) {
             L.claims = 0;
             L.minv = 0xffffffffu;
@@ -1164,6 +1175,8 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         a.stats[KC_ST_KMAX] = kmax;
         a.stats[KC_ST_PHASES] = gen;
         a.stats[KC_ST_LEVELS_RUN] = levels_run;
+        a.stats[KC_ST_TAIL_LEVEL] = tail_level;
+        a.stats[KC_ST_LIVE] = a.V - total;
         a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
     }
     uint64_t ew = wave_sum_u64(edges);
@@ -1243,6 +1256,8 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.out = p.out;
     a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
+    a.tail_live = p.mode == MODE_BUCKET ? p.tail_live : 0u;
+    a.start_level = p.mode == MODE_BUCKET ? p.start_level : 0u;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;

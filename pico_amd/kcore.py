@@ -37,7 +37,8 @@ class Opts(ctypes.Structure):
                 ("hub_split", ctypes.c_uint32), ("queue_cap", ctypes.c_uint32),
                 ("gpp_compat", ctypes.c_uint32), ("part_size", ctypes.c_uint32),
                 ("local_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("agg_div", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("agg_div", ctypes.c_uint32), ("compact_div", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -45,7 +46,7 @@ class Stats(ctypes.Structure):
                 ("levels_run", ctypes.c_uint64), ("phases", ctypes.c_uint64),
                 ("edges_walked", ctypes.c_uint64), ("kmax", ctypes.c_uint32),
                 ("grid", ctypes.c_uint32), ("block", ctypes.c_uint32),
-                ("reserved0", ctypes.c_uint32), ("device_ms", ctypes.c_double),
+                ("stages", ctypes.c_uint32), ("device_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
                 ("prep_ms", ctypes.c_double), ("changed_degree", ctypes.c_uint64),
                 ("atomics", ctypes.c_uint64)]
@@ -514,7 +515,7 @@ class DeviceGraph:
         ms = ctypes.c_double(0)
         _check(lib().kc_autotune(self._h, m, reps, ctypes.byref(o), ctypes.byref(ms)))
         return {"block": o.block, "hub_split": o.hub_split, "local_cap": o.local_cap,
-                "agg_div": o.agg_div}, ms.value
+                "agg_div": o.agg_div, "compact_div": o.compact_div}, ms.value
 
     def __init__(self, ctx, handle, V):
         self.ctx = ctx

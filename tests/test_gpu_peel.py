@@ -162,3 +162,34 @@ def test_atomic_count_bounds(ctx, method):
     assert st["edges_walked"] > 0
     assert 0 < st["atomics"] <= 3 * st["edges_walked"] + hg.V
     assert plain["atomics"] == 0  # the default variant does not count
+
+
+@pytest.mark.parametrize("method", ["peel_dyn", "peelone_dyn"])
+@pytest.mark.parametrize("div", [2, 4, 16])
+@pytest.mark.parametrize("gen", ["rmat15", "chunglu_wide"])
+def test_compacted_stages(ctx, method, div, gen):
+    # the bucket peels continue on compacted live subgraphs: same cores, same
+    # visited levels as without compaction, fewer adjacency entries walked
+    if gen == "rmat15":
+        hg = pa.HostGraph.rmat(15, 16, seed=9)
+        dg = ctx.upload(hg)
+    else:
+        hg = pa.HostGraph.chunglu(1 << 15, beta=2.1, mean_deg=20.0, cap=4000.0, seed=4)
+        dg = ctx.upload(hg, rowptr_bytes=8)
+    try:
+        core, st = dg.decompose(method, compact_div=div)
+        ref, st0 = dg.decompose(method, compact_div=0xFFFFFFFF)
+        for block in (256, 1024):
+            again, _ = dg.decompose(method, compact_div=div, block=block)
+            np.testing.assert_array_equal(again, core)
+    finally:
+        dg.close()
+    want = oracle.bz(hg.rowptr, hg.col)
+    np.testing.assert_array_equal(core, want)
+    np.testing.assert_array_equal(ref, want)
+    assert st["iterations"] == st0["iterations"] == int(want.max()) + 1
+    assert st["levels_run"] == st0["levels_run"]
+    assert st["edges_walked"] <= st0["edges_walked"]
+    assert st0["stages"] == 1
+    if div <= 4:  # V/div >= 4,096 live vertices: at least one compaction
+        assert st["stages"] >= 2
