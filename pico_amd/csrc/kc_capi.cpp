@@ -552,6 +552,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     uint64_t acc_levels = 0, acc_phases = 0, acc_edges = 0, acc_atomics = 0;
     bool compacted = false;
     uint32_t nstages = 1;
+    uint64_t stage_nnz = g->nnz; // bound on the current stage's adjacency
     for (int stage = 0;; stage++) {
         KC_HIP(hipMemcpyAsync(sv, g->stats.p, sizeof sv, hipMemcpyDeviceToHost, c->stream),
                "stats copy");
@@ -561,6 +562,11 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         if (sv[kc::KC_ST_ERR] == 2)
             return fail(KC_EOVERFLOW, "shared work list overflow (capacity %u items)",
                         p.list_cap);
+        if (sv[kc::KC_ST_ERR] == 4) {
+            uint32_t bad = 0;
+            KC_HIP(hipMemcpy(&bad, g->cs_bad.p, 4, hipMemcpyDeviceToHost), "compaction copy");
+            return fail(KC_EDEVICE, "live-subgraph compaction failed its checks (flags %u)", bad);
+        }
         if (sv[kc::KC_ST_ERR])
             return fail(KC_EDEVICE, "device consistency check failed (%llu)",
                         (unsigned long long)sv[kc::KC_ST_ERR]);
@@ -586,14 +592,17 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
             (s = ensure(g, g->cs_gmap[nxt], (size_t)Vn * 4 + 4, "compaction")) ||
             (s = ensure(g, g->cs_row[nxt], (size_t)(Vn + 1) * rb, "compaction")) ||
             (s = ensure(g, g->cs_tmp, tb, "compaction")) ||
-            (s = ensure(g, g->cs_bad, 4, "compaction")))
+            (s = ensure(g, g->cs_bad, 8, "compaction")))
             return s;
-        if (!compacted)
-            KC_HIP(hipMemsetAsync(g->cs_bad.p, 0, 4, c->stream), "memset");
+        // flags (kept across stages; checked after each) and the long-row count
+        KC_HIP(hipMemsetAsync((char *)g->cs_bad.p + (compacted ? 4 : 0), 0, compacted ? 4 : 8,
+                              c->stream),
+               "memset");
         compacted = true;
         kc::CompactIn in{p.rowptr, p.wide, p.col, p.V, p.deg, p.done,
                          stage > 0 ? (const uint32_t *)g->cs_gmap[cur].p : nullptr};
         kc::CompactOut co{};
+        co.V_in = p.V;
         co.Vn = Vn;
         co.wpre = (uint32_t *)g->cs_wpre.p;
         co.loc = (uint32_t *)g->cs_loc.p;
@@ -603,32 +612,21 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         co.bad = (uint32_t *)g->cs_bad.p;
         co.tmp = g->cs_tmp.p;
         co.tmp_bytes = g->cs_tmp.n;
-        KC_HIP(kc::compact_rows(in, co, c->stream), "compaction rows");
-        uint32_t live_chk = 0, bad = 0;
-        uint64_t nnz_n = 0;
-        KC_HIP(hipMemcpyAsync(&live_chk, co.wpre + nw, 4, hipMemcpyDeviceToHost, c->stream),
-               "compaction copy");
-        KC_HIP(hipMemcpyAsync(&nnz_n, (char *)co.rowptr + (size_t)Vn * rb, rb,
-                              hipMemcpyDeviceToHost, c->stream),
-               "compaction copy");
-        KC_HIP(hipMemcpyAsync(&bad, co.bad, 4, hipMemcpyDeviceToHost, c->stream),
-               "compaction copy");
-        KC_HIP(hipStreamSynchronize(c->stream), "compaction rows");
-        if (live_chk != Vn || bad)
-            return fail(KC_EDEVICE, "compaction: %u live vertices counted, %u expected (flags %u)",
-                        live_chk, Vn, bad);
-        if ((s = ensure(g, g->cs_col[nxt], (size_t)nnz_n * 4 + 4, "compaction")) ||
+        // No host round trip: the adjacency gets an upper bound of room, the
+        // copy kernels check the renumbering and every row against it, and
+        // the next stage leaves at once (error 4) if any check failed.
+        const uint64_t nnz_cap = std::min<uint64_t>(stage_nnz, (uint64_t)Vn * Vn);
+        if ((s = ensure(g, g->cs_col[nxt], (size_t)nnz_cap * 4 + 4, "compaction")) ||
             (s = ensure(g, g->cs_deg[nxt], (size_t)Vn * 4 + 4, "compaction")) ||
             (s = ensure(g, g->cs_done[nxt], (size_t)((Vn + 63) / 64) * 8 + 8, "compaction")))
             return s;
         co.col = (uint32_t *)g->cs_col[nxt].p;
+        co.col_cap = nnz_cap;
+        KC_HIP(kc::compact_rows(in, co, c->stream), "compaction rows");
         KC_HIP(kc::compact_cols(in, co, c->stream), "compaction adjacency");
-        // every row must be complete before the next stage reads it
-        KC_HIP(hipMemcpyAsync(&bad, co.bad, 4, hipMemcpyDeviceToHost, c->stream),
-               "compaction copy");
-        KC_HIP(hipStreamSynchronize(c->stream), "compaction adjacency");
-        if (bad)
-            return fail(KC_EDEVICE, "compaction: row lengths differ from degrees (flags %u)", bad);
+        const uint64_t nnz_n = nnz_cap;
+        stage_nnz = nnz_cap;
+        p.abort_flag = co.bad;
         // the next stage: same launch shape, the level where this one stopped
         p.rowptr = co.rowptr;
         p.wide = wide_n;

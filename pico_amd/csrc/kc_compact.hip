@@ -39,7 +39,8 @@ __global__ void live_word_count(const uint64_t *done, uint32_t nw, uint32_t *cnt
 template <typename R>
 __global__ void renumber(const R *rowptr, const uint64_t *done, const uint32_t *wpre,
                          const uint32_t *deg, const uint32_t *gmap_in, uint32_t V, uint32_t Vn,
-                         uint32_t *loc, uint32_t *gmap_out, uint32_t *rowlen, uint32_t *bad)
+                         uint32_t *loc, uint32_t *gmap_out, uint32_t *rowlen, uint32_t *bad,
+                         uint32_t *longs)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0)
@@ -64,97 +65,169 @@ __global__ void renumber(const R *rowptr, const uint64_t *done, const uint32_t *
         d = 0;
     }
     rowlen[i] = d;
+    if (full > LONG_ROW) // listed for copy_long (bad[1] counts them)
+        longs[atomicAdd(&bad[1], 1u)] = i;
+}
+
+// Per 64 vertices: the free (live) bits and the live vertices before them,
+// side by side so that renumbering a neighbour is one 16-byte gather.
+struct alignas(16) LiveWord {
+    uint64_t freew;
+    uint32_t before;
+    uint32_t pad;
+};
+
+__global__ void pack_words(const uint64_t *done, const uint32_t *wpre, uint32_t nw, LiveWord *lw)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw)
+        lw[i] = LiveWord{~done[i], wpre[i], 0u};
 }
 
 // New id of u if live, else ~0.
-__device__ __forceinline__ uint32_t new_id(const uint64_t *done, const uint32_t *wpre, uint32_t u)
+__device__ __forceinline__ uint32_t new_id(const LiveWord *lw, uint32_t u)
 {
-    uint64_t freew = ~done[u >> 6];
+    LiveWord x = lw[u >> 6];
     uint32_t b = u & 63;
-    if (!((freew >> b) & 1ull))
+    if (!((x.freew >> b) & 1ull))
         return 0xffffffffu;
-    return wpre[u >> 6] + (uint32_t)__popcll(freew & ((1ull << b) - 1ull));
+    return x.before + (uint32_t)__popcll(x.freew & ((1ull << b) - 1ull));
 }
 
-// One wave per short row.
+// One wave per short row, 4 x 64 entries in flight per step.  A wave takes
+// SR consecutive rows at a time and loads their bounds with one access per
+// lane, so a row costs only its own adjacency's round trips.
+constexpr uint32_t SR = 8;
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, uint32_t src)
+{
+    uint32_t lo = __shfl((uint32_t)x, src, KC_WAVE), hi = __shfl((uint32_t)(x >> 32), src, KC_WAVE);
+    return (uint64_t)hi << 32 | lo;
+}
+
 template <typename R, typename R2>
 __global__ __launch_bounds__(CB) void copy_short(const R *rowptr, const uint32_t *col,
-                                                 const uint64_t *done, const uint32_t *wpre,
-                                                 const uint32_t *loc, uint32_t Vn,
-                                                 const R2 *orow, uint32_t *ocol, uint32_t *bad)
+                                                 const LiveWord *lw, const uint32_t *wpre,
+                                                 const uint32_t *loc, uint32_t Vn, uint32_t nw,
+                                                 const R2 *orow, uint32_t *ocol, uint64_t cap,
+                                                 uint32_t *bad)
 {
+    constexpr uint32_t U = 4;
     const uint32_t lane = lane_id();
+    if (wpre[nw] != Vn) { // the renumbering is not a bijection onto [0, Vn)
+        if (threadIdx.x == 0)
+            atomicOr(bad, 16u);
+        return;
+    }
     const uint32_t nwaves = gridDim.x * (CB / KC_WAVE);
-    for (uint32_t i = blockIdx.x * (CB / KC_WAVE) + threadIdx.x / KC_WAVE; i < Vn; i += nwaves) {
-        uint32_t v = loc[i];
-        uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
-        if (e - s > LONG_ROW)
-            continue;
-        uint64_t ob = (uint64_t)orow[i], oe = (uint64_t)orow[i + 1];
-        uint64_t pos = ob;
-        for (uint64_t j0 = s; j0 < e; j0 += KC_WAVE) {
-            uint64_t j = j0 + lane;
-            uint32_t id = j < e ? new_id(done, wpre, col[j]) : 0xffffffffu;
-            uint64_t m = ballot(id != 0xffffffffu);
-            uint64_t p = pos + lane_rank(m);
-            if (id != 0xffffffffu && p < oe)
-                ocol[p] = id;
-            pos += (uint32_t)__popcll(m);
+    for (uint32_t i0 = (blockIdx.x * (CB / KC_WAVE) + threadIdx.x / KC_WAVE) * SR; i0 < Vn;
+         i0 += nwaves * SR) {
+        uint32_t il = i0 + lane;
+        uint64_t ls = 0, le = 0, lob = 0, loe = 0;
+        if (lane < SR && il < Vn) {
+            uint32_t v = loc[il];
+            ls = (uint64_t)rowptr[v];
+            le = (uint64_t)rowptr[v + 1];
+            lob = (uint64_t)orow[il];
+            loe = (uint64_t)orow[il + 1];
         }
-        if (lane == 0 && pos != oe)
-            atomicOr(bad, 4u);
+        const uint32_t nr = Vn - i0 < SR ? Vn - i0 : SR;
+        for (uint32_t r = 0; r < nr; r++) {
+            uint64_t s = shfl64(ls, r), e = shfl64(le, r);
+            if (e - s > LONG_ROW)
+                continue;
+            uint64_t pos = shfl64(lob, r), oe = shfl64(loe, r);
+            if (oe > cap) {
+                if (lane == 0)
+                    atomicOr(bad, 32u);
+                continue;
+            }
+            for (uint64_t j0 = s; j0 < e; j0 += (uint64_t)KC_WAVE * U) {
+                uint32_t u[U], id[U];
+#pragma unroll
+                for (uint32_t k = 0; k < U; k++) {
+                    uint64_t j = j0 + (uint64_t)k * KC_WAVE + lane;
+                    u[k] = j < e ? col[j] : 0xffffffffu;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < U; k++)
+                    id[k] = u[k] != 0xffffffffu ? new_id(lw, u[k]) : 0xffffffffu;
+#pragma unroll
+                for (uint32_t k = 0; k < U; k++) {
+                    uint64_t m = ballot(id[k] != 0xffffffffu);
+                    uint64_t p = pos + lane_rank(m);
+                    if (id[k] != 0xffffffffu && p < oe)
+                        ocol[p] = id[k];
+                    pos += (uint32_t)__popcll(m);
+                }
+            }
+            if (lane == 0 && pos != oe)
+                atomicOr(bad, 4u);
+        }
     }
 }
 
-// One workgroup per long row, 4 entries per thread per step.
+// One workgroup per long row, 16 entries per thread per step.
 template <typename R, typename R2>
 __global__ __launch_bounds__(CB) void copy_long(const R *rowptr, const uint32_t *col,
-                                                const uint64_t *done, const uint32_t *wpre,
-                                                const uint32_t *loc, uint32_t Vn, const R2 *orow,
-                                                uint32_t *ocol, uint32_t *bad)
+                                                const LiveWord *lw, const uint32_t *wpre,
+                                                const uint32_t *loc, const uint32_t *longs,
+                                                uint32_t Vn, uint32_t nw, const R2 *orow,
+                                                uint32_t *ocol, uint64_t cap, uint32_t *bad)
 {
-    constexpr uint32_t U = 4;
+    if (wpre[nw] != Vn)
+        return; // reported by copy_short
+    const uint32_t nlong = bad[1];
+    constexpr uint32_t U = 16;
     constexpr uint32_t NW = CB / KC_WAVE;
     __shared__ uint32_t s_w[NW * U];
+    __shared__ uint32_t s_pre[NW * U + 1];
     const uint32_t lane = lane_id(), w = threadIdx.x / KC_WAVE;
-    for (uint32_t i = blockIdx.x; i < Vn; i += gridDim.x) {
+    for (uint32_t li = blockIdx.x; li < nlong; li += gridDim.x) {
+        uint32_t i = longs[li];
         uint32_t v = loc[i];
         uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
-        if (e - s <= LONG_ROW)
-            continue;
         uint64_t oe = (uint64_t)orow[i + 1];
         uint64_t pos = (uint64_t)orow[i];
+        if (oe > cap) {
+            if (threadIdx.x == 0)
+                atomicOr(bad, 32u);
+            continue;
+        }
         for (uint64_t j0 = s; j0 < e; j0 += (uint64_t)CB * U) {
             uint32_t id[U];
-            uint64_t m[U];
 #pragma unroll
             for (uint32_t k = 0; k < U; k++) {
                 uint64_t j = j0 + (uint64_t)k * CB + threadIdx.x;
-                id[k] = j < e ? new_id(done, wpre, col[j]) : 0xffffffffu;
+                id[k] = j < e ? col[j] : 0xffffffffu;
             }
 #pragma unroll
+            for (uint32_t k = 0; k < U; k++)
+                id[k] = id[k] != 0xffffffffu ? new_id(lw, id[k]) : 0xffffffffu;
+#pragma unroll
             for (uint32_t k = 0; k < U; k++) {
-                m[k] = ballot(id[k] != 0xffffffffu);
+                uint64_t m = ballot(id[k] != 0xffffffffu);
                 if (lane == 0)
-                    s_w[k * NW + w] = (uint32_t)__popcll(m[k]);
+                    s_w[k * NW + w] = (uint32_t)__popcll(m);
             }
             __syncthreads();
             // entries are ordered by (k, wave, lane): prefix over s_w in that order
-            uint32_t before = 0, total = 0;
-            for (uint32_t t = 0; t < NW * U; t++) {
-                uint32_t c = s_w[t];
-                total += c;
+            if (threadIdx.x == 0) {
+                uint32_t run = 0;
+                for (uint32_t t = 0; t < NW * U; t++) {
+                    s_pre[t] = run;
+                    run += s_w[t];
+                }
+                s_pre[NW * U] = run;
             }
+            __syncthreads();
 #pragma unroll
             for (uint32_t k = 0; k < U; k++) {
-                before = 0;
-                for (uint32_t t = 0; t < k * NW + w; t++)
-                    before += s_w[t];
-                uint64_t p = pos + before + lane_rank(m[k]);
+                uint64_t m = ballot(id[k] != 0xffffffffu);
+                uint64_t p = pos + s_pre[k * NW + w] + lane_rank(m);
                 if (id[k] != 0xffffffffu && p < oe)
                     ocol[p] = id[k];
             }
-            pos += total;
+            pos += s_pre[NW * U];
             __syncthreads();
         }
         if (threadIdx.x == 0 && pos != oe)
@@ -172,6 +245,18 @@ __global__ void scatter_kernel(const uint32_t *sub, const uint32_t *gmap, uint32
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// scratch layout: word counts | row lengths | long rows | packed words | scan
+inline uint32_t *longs_of(const CompactOut &out)
+{
+    uint32_t nw = (out.V_in + 63) / 64;
+    return (uint32_t *)((char *)out.tmp + align256((size_t)(nw + 1) * 4) +
+                        align256((size_t)(out.Vn + 1) * 4));
+}
+inline LiveWord *words_of(const CompactOut &out)
+{
+    return (LiveWord *)((char *)longs_of(out) + align256((size_t)(out.Vn + 1) * 4));
+}
+
 template <typename R2> size_t scan_bytes(uint32_t n)
 {
     size_t b = 0;
@@ -187,8 +272,8 @@ size_t compact_tmp_bytes(uint32_t V, uint32_t Vn)
     size_t a = scan_bytes<uint32_t>(nw + 1);
     size_t b = scan_bytes<uint64_t>(Vn + 1);
     size_t c = scan_bytes<uint32_t>(Vn + 1);
-    return align256((size_t)(nw + 1) * 4) + align256((size_t)(Vn + 1) * 4) +
-           align256(std::max(a, std::max(b, c)));
+    return align256((size_t)(nw + 1) * 4) + 2 * align256((size_t)(Vn + 1) * 4) +
+           align256((size_t)(nw + 1) * sizeof(LiveWord)) + align256(std::max(a, std::max(b, c)));
 }
 
 hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st)
@@ -197,22 +282,25 @@ hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st)
     char *t = (char *)out.tmp;
     uint32_t *cnt = (uint32_t *)t;
     uint32_t *rowlen = (uint32_t *)(t + align256((size_t)(nw + 1) * 4));
-    void *ctmp = t + align256((size_t)(nw + 1) * 4) + align256((size_t)(out.Vn + 1) * 4);
+    uint32_t *longs = longs_of(out);
+    void *ctmp = (char *)words_of(out) + align256((size_t)(nw + 1) * sizeof(LiveWord));
     size_t cbytes = out.tmp_bytes - (size_t)((char *)ctmp - t);
     hipLaunchKernelGGL(live_word_count, dim3((nw + 1 + CB - 1) / CB), dim3(CB), 0, st, in.done,
                        nw, cnt);
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(ctmp, cbytes, cnt, out.wpre, nw + 1, st);
     if (e != hipSuccess)
         return e;
+    hipLaunchKernelGGL(pack_words, dim3((nw + CB - 1) / CB), dim3(CB), 0, st, in.done, out.wpre,
+                       nw, words_of(out));
     if (in.V) {
         if (in.wide)
             hipLaunchKernelGGL(renumber<uint64_t>, dim3((in.V + CB - 1) / CB), dim3(CB), 0, st,
                                (const uint64_t *)in.rowptr, in.done, out.wpre, in.deg, in.gmap,
-                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad);
+                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad, longs);
         else
             hipLaunchKernelGGL(renumber<uint32_t>, dim3((in.V + CB - 1) / CB), dim3(CB), 0, st,
                                (const uint32_t *)in.rowptr, in.done, out.wpre, in.deg, in.gmap,
-                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad);
+                               in.V, out.Vn, out.loc, out.gmap, rowlen, out.bad, longs);
     }
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
@@ -226,14 +314,16 @@ hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st)
 template <typename R, typename R2>
 static void launch_copy(const CompactIn &in, const CompactOut &out, hipStream_t st)
 {
-    uint32_t gs = std::min<uint32_t>(4096, (out.Vn + 3) / 4 + 1);
+    uint32_t gs = std::min<uint32_t>(16384, (out.Vn + SR * (CB / KC_WAVE) - 1) /
+                                                (SR * (CB / KC_WAVE)) + 1);
+    const uint32_t nw = (in.V + 63) / 64;
     hipLaunchKernelGGL((copy_short<R, R2>), dim3(gs), dim3(CB), 0, st, (const R *)in.rowptr,
-                       in.col, in.done, out.wpre, out.loc, out.Vn, (const R2 *)out.rowptr,
-                       out.col, out.bad);
+                       in.col, (const LiveWord *)words_of(out), out.wpre, out.loc, out.Vn, nw, (const R2 *)out.rowptr,
+                       out.col, out.col_cap, out.bad);
     uint32_t gl = std::min<uint32_t>(2048, out.Vn ? out.Vn : 1);
     hipLaunchKernelGGL((copy_long<R, R2>), dim3(gl), dim3(CB), 0, st, (const R *)in.rowptr, in.col,
-                       in.done, out.wpre, out.loc, out.Vn, (const R2 *)out.rowptr, out.col,
-                       out.bad);
+                       (const LiveWord *)words_of(out), out.wpre, out.loc, (const uint32_t *)longs_of(out), out.Vn, nw,
+                       (const R2 *)out.rowptr, out.col, out.col_cap, out.bad);
 }
 
 hipError_t compact_cols(const CompactIn &in, const CompactOut &out, hipStream_t st)

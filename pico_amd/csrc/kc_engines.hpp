@@ -56,6 +56,7 @@ struct PeelLaunch {
     uint32_t agg_live;    // pre-check degrees and aggregate hub decrements once <= this many remain
     uint32_t tail_live;   // bucket: stop at a level start once <= this many remain (0: never)
     uint32_t start_level; // bucket: first level to visit
+    const uint32_t *abort_flag; // device word; non-zero: the kernel leaves at once (error 4)
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;
@@ -82,6 +83,7 @@ struct CompactIn {
     const uint32_t *gmap;  // stage id -> original id, or null for the original graph
 };
 struct CompactOut {
+    uint32_t V_in;        // vertices of the stage graph
     uint32_t Vn;          // live vertices (known from the peel's stats)
     uint32_t *wpre;       // (V+63)/64 + 1 words: live vertices before each claim word
     uint32_t *loc;        // Vn: stage id of each new vertex
@@ -89,14 +91,17 @@ struct CompactOut {
     void *rowptr;         // Vn+1, u32 (wide == false) or u64
     bool wide;
     uint32_t *col;        // rowptr[Vn] entries
-    uint32_t *bad;        // set non-zero if a row's live count differs from its degree
+    uint64_t col_cap;     // entries allocated at col (rows reaching past it are refused)
+    uint32_t *bad;        // [0] set non-zero if a row's live count differs from its degree,
+                          // [1] long rows listed (zero before compact_rows)
     void *tmp;            // scan scratch
     size_t tmp_bytes;
 };
-// Phase 1: renumbering and row starts; *nnz_dev receives rowptr[Vn].
+// Phase 1: renumbering and row starts.
 hipError_t compact_rows(const CompactIn &in, CompactOut &out, hipStream_t st);
 size_t compact_tmp_bytes(uint32_t V, uint32_t Vn);
-// Phase 2: the adjacency.
+// Phase 2: the adjacency.  A failed consistency check sets bad[0]; the next
+// peel stage takes bad as its abort flag.
 hipError_t compact_cols(const CompactIn &in, const CompactOut &out, hipStream_t st);
 // core[gmap[i]] = sub[i] for i < n
 hipError_t scatter_cores(const uint32_t *sub, const uint32_t *gmap, uint32_t n, uint32_t *core,

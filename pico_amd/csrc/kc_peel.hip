@@ -43,6 +43,7 @@ template <typename R> struct PeelArgs {
     uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
     uint32_t tail_live;   // bucket: stop at a level start once <= this many are live
     uint32_t start_level; // bucket: first level (every vertex's degree is >= it)
+    const uint32_t *abort_flag; // non-zero: the inputs are unusable, do nothing
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional: KC_TRACE_WORDS per phase, see kc_engines.hpp
@@ -967,7 +968,7 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 }
 
 // phase kinds of the persistent loop
-enum { PH_SCAN = 0, PH_SHARED = 1, PH_GPP_SCATTER = 2, PH_GPP_CROSS = 3 };
+enum { PH_SCAN = 0, PH_SHARED = 1, PH_GPP_SCATTER = 2, PH_GPP_CROSS = 3, PH_DONE = 4 };
 
 template <int BLOCK, int LQ, typename R, bool CNT>
 __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
@@ -1014,7 +1015,14 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         L.agg_key[s] = KC_DEAD;
         L.agg_cnt[s] = 0;
     }
-    for (;;) {
+    // written by an earlier kernel on the stream: every workgroup reads the
+    // same value and leaves before touching the graph
+    if (a.abort_flag && *a.abort_flag) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            sync_fail(gs, 4);
+        kind = PH_DONE;
+    }
+    while (kind != PH_DONE) {
         // Few vertices left: stop before this level's scan so the caller can
         // compact the live subgraph and continue on it (the state is that of
         // a completed level; every workgroup takes the same decision).
@@ -1257,6 +1265,7 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
     a.tail_live = p.mode == MODE_BUCKET ? p.tail_live : 0u;
+    a.abort_flag = p.abort_flag;
     a.start_level = p.mode == MODE_BUCKET ? p.start_level : 0u;
     a.gs = p.gs;
     a.stats = p.stats;
