@@ -98,10 +98,15 @@ __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
 // ----------------------------------------------------------- grid sync ----
 // Reduction slots carried by the barrier.  Phase g (the work between barrier
-// g-1 and barrier g) accumulates into slot[g & 1]; the last WG to arrive at
-// barrier g clears slot[(g+1) & 1] before releasing, so every slot is clean
-// before its phase starts and stays readable until the following barrier.
-struct PhaseSlot {
+// g-1 and barrier g) accumulates into slot[g % 3], which is read after
+// barrier g and during phase g+1; the last WG to arrive at barrier g clears
+// slot[(g + 2) % 3] (last read in phase g) after releasing the others, off
+// the barrier's critical path, and its own wait at barrier g+1 completes
+// those stores before phase g+2 uses the slot.
+// Each slot, the arrival counter and the generation word sit on cache lines
+// of their own (128 bytes), so that polling one does not contend with
+// atomics on another.
+struct alignas(128) PhaseSlot {
     uint64_t sum0;  // engine-defined sums
     uint64_t sum1;
     uint64_t sum2;
@@ -119,16 +124,22 @@ struct PhaseSlot {
 };
 
 struct GridSync {
-    uint32_t arrive;
-    uint32_t gen;
-    uint32_t err;      // 0 ok, 1 spin timeout, 2 queue overflow
+    alignas(128) uint32_t arrive; // arrivals since the launch: barrier g is complete
+                                  // at (g+1) * nblocks
     uint32_t nblocks;
     uint64_t timeout_ticks;
     uint64_t t0;
-    uint32_t live_next; // reservations of the peel's live-vertex list
+    alignas(128) uint32_t gen; // barriers completed (polled by the waiting workgroups)
+    uint32_t err;              // 0 ok, 1 spin timeout, 2 queue overflow, ...
+    alignas(128) uint32_t live_next; // reservations of the peel's live-vertex list
     uint32_t pad;
-    PhaseSlot slot[2];
+    PhaseSlot slot[3];
 };
+
+__device__ __forceinline__ PhaseSlot *slot_of(GridSync *gs, uint32_t phase)
+{
+    return &gs->slot[phase % 3];
+}
 
 __device__ __forceinline__ void slot_clear(PhaseSlot *s, uint32_t nblocks)
 {
@@ -140,6 +151,27 @@ __device__ __forceinline__ void slot_clear(PhaseSlot *s, uint32_t nblocks)
     st_relaxed(&s->qtail, 0u);
     st_relaxed(&s->qhead, 0u);
     st_relaxed(&s->flag, 0u);
+}
+
+// Called by one thread of the kernel that precedes a persistent launch.
+__device__ __forceinline__ void grid_sync_init(GridSync *gs, uint32_t nblocks, uint64_t timeout)
+{
+    gs->arrive = 0;
+    gs->gen = 0;
+    gs->err = 0;
+    gs->nblocks = nblocks;
+    gs->timeout_ticks = timeout;
+    gs->t0 = 0;
+    gs->live_next = 0;
+    gs->pad = 0;
+    for (int p = 0; p < 3; p++) {
+        PhaseSlot *s = &gs->slot[p];
+        s->sum0 = s->sum1 = s->sum2 = 0;
+        s->min0 = 0xffffffffu;
+        s->qtail = s->qhead = s->flag = 0;
+        s->pad0 = s->pad1 = 0;
+        s->qword = nblocks;
+    }
 }
 
 __device__ __forceinline__ bool sync_failed(GridSync *gs)
@@ -173,11 +205,10 @@ __device__ __forceinline__ bool grid_barrier(GridSync *gs, uint32_t &gen)
             __threadfence();
         uint32_t nb = gs->nblocks;
         uint32_t arrived = atomicAdd(&gs->arrive, 1u);
-        if (arrived == nb - 1) {
-            st_relaxed(&gs->arrive, 0u);
-            slot_clear(&gs->slot[(gen + 1) & 1], nb);
-            vm_drain();
+        if (arrived == (gen + 1) * nb - 1) {
+            // every workgroup's stores completed before its arrival
             atomicAdd(&gs->gen, 1u);
+            slot_clear(&gs->slot[(gen + 2) % 3], nb);
         } else {
             uint64_t start = now_ticks();
             while (ld_relaxed(&gs->gen) == gen) {

@@ -120,7 +120,7 @@ __device__ __forceinline__ void hpush(GridSync *gs, uint32_t phase, uint32_t *ls
     uint32_t leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
     if (lane == leader)
-        base = atomicAdd(&gs->slot[phase & 1].qtail, (uint32_t)__popcll(m));
+        base = atomicAdd(&slot_of(gs, phase)[0].qtail, (uint32_t)__popcll(m));
     base = __shfl(base, leader, KC_WAVE);
     if (want) {
         uint32_t p = base + lane_rank(m);
@@ -343,7 +343,7 @@ __device__ void for_items_g(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, u
     for (;;) {
         uint32_t base = 0;
         if (lane == 0)
-            base = atomicAdd(&gs->slot[phase & 1].qhead, KC_HGRAB);
+            base = atomicAdd(&slot_of(gs, phase)[0].qhead, KC_HGRAB);
         base = __shfl(base, 0, KC_WAVE);
         if (base >= n)
             break;
@@ -405,7 +405,7 @@ __device__ __forceinline__ bool hbar(const HArgs<R> &a, GridSync *gs, uint32_t &
         uint64_t t = now_ticks();
         tr[2] = t;
         tr[4] = round;
-        tr[5] = ld_relaxed(&gs->slot[phase & 1].qtail);
+        tr[5] = ld_relaxed(&slot_of(gs, phase)[0].qtail);
         tr[7] = kind;
         if (phase + 1 < a.trace_cap)
             a.trace[(uint64_t)(phase + 1) * KC_TRACE_WORDS] = t;
@@ -530,8 +530,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 __builtin_amdgcn_wave_barrier();
                 uint32_t base = 0;
                 if (lane == 0) {
-                    atomicAdd((unsigned long long *)&gs->slot[phase & 1].sum0, (unsigned long long)n_st);
-                    base = atomicAdd(&gs->slot[phase & 1].qtail, st_nch);
+                    atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0, (unsigned long long)n_st);
+                    base = atomicAdd(&slot_of(gs, phase)[0].qtail, st_nch);
                 }
                 base = __shfl(base, 0, KC_WAVE);
                 if (base + st_nch > a.chunk_cap) {
@@ -572,9 +572,9 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                             a.histo[b + k] = sum;
                                             a.core[v] = k;
                                             chg_deg += d;
-                                            atomicAdd((unsigned long long *)&gs->slot[phase & 1].sum0,
+                                            atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0,
                                                       1ull);
-                                            L.bk = atomicAdd(&gs->slot[phase & 1].qtail, nch);
+                                            L.bk = atomicAdd(&slot_of(gs, phase)[0].qtail, nch);
                                         }
                                         __syncthreads();
                                         uint32_t base = L.bk;
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     __builtin_amdgcn_wave_barrier();
                     uint32_t base = 0;
                     if (lane == 0)
-                        base = atomicAdd(&gs->slot[phase & 1].qtail, n_st);
+                        base = atomicAdd(&slot_of(gs, phase)[0].qtail, n_st);
                     base = __shfl(base, 0, KC_WAVE);
                     if (lane < n_st) {
                         if (base + lane < cap_list)
@@ -656,9 +656,9 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         // HistoCore counts changed vertices in sum0 and queues their virtual
         // nodes on the chunk list; the others list changed vertices
         uint32_t n_changed = a.method == HM_HISTO
-                                 ? (uint32_t)ld_relaxed64(&gs->slot[phase & 1].sum0)
-                                 : ld_relaxed(&gs->slot[phase & 1].qtail);
-        uint32_t n_chunks = ld_relaxed(&gs->slot[phase & 1].qtail);
+                                 ? (uint32_t)ld_relaxed64(&slot_of(gs, phase)[0].sum0)
+                                 : ld_relaxed(&slot_of(gs, phase)[0].qtail);
+        uint32_t n_chunks = ld_relaxed(&slot_of(gs, phase)[0].qtail);
         changed_total += n_changed;
         if (n_changed == 0)
             break;
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             for (;;) {
                 uint32_t base = 0;
                 if (lane == 0)
-                    base = atomicAdd(&gs->slot[phase & 1].qhead, KC_HGRAB);
+                    base = atomicAdd(&slot_of(gs, phase)[0].qhead, KC_HGRAB);
                 base = __shfl(base, 0, KC_WAVE);
                 if (base >= n_chunks)
                     break;
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             if (!hbar(a, gs, gen, round, 2))
                 goto out;
             if (threadIdx.x == 0)
-                L.done = ld_relaxed(&gs->slot[phase & 1].qtail);
+                L.done = ld_relaxed(&slot_of(gs, phase)[0].qtail);
             __syncthreads();
             continue;
         }
@@ -799,7 +799,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 });
             if (!hbar(a, gs, gen, round, 2))
                 goto out;
-            uint32_t n_marked = ld_relaxed(&gs->slot[phase & 1].qtail);
+            uint32_t n_marked = ld_relaxed(&slot_of(gs, phase)[0].qtail);
             if (a.method == HM_NBR) {
                 if (threadIdx.x == 0)
                     L.done = n_marked;
@@ -861,7 +861,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             }
             if (!hbar(a, gs, gen, round, 3))
                 goto out;
-            uint32_t n_act = ld_relaxed(&gs->slot[phase & 1].qtail);
+            uint32_t n_act = ld_relaxed(&slot_of(gs, phase)[0].qtail);
             act_cur = act_out;
             act_nxt = act_in;
             n_act_prev = n_act;
@@ -903,17 +903,7 @@ __global__ void hindex_init_kernel(const R *rowptr, uint32_t V, uint32_t *core, 
             mark[v] = 0;
     }
     if (i == 0) {
-        gs->arrive = 0;
-        gs->gen = 0;
-        gs->err = 0;
-        gs->nblocks = nblocks;
-        gs->timeout_ticks = timeout;
-        gs->t0 = 0;
-        for (int p = 0; p < 2; p++) {
-            gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
-            gs->slot[p].min0 = 0xffffffffu;
-            gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
-        }
+        grid_sync_init(gs, nblocks, timeout);
         for (int k = 0; k < KC_ST_COUNT; k++)
             stats[k] = 0;
     }

@@ -117,7 +117,7 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
     if (!m)
         return;
     uint32_t lane = lane_id();
-    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    PhaseSlot *slot = &slot_of(a.gs, phase)[0];
     uint64_t *list = (phase & 1) ? a.list1 : a.list0;
     uint32_t leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
@@ -190,7 +190,7 @@ __device__ __forceinline__ void push_hub_chunks(const PeelArgs<R> &a, uint32_t p
 {
     if (nch <= 1)
         return;
-    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    PhaseSlot *slot = &slot_of(a.gs, phase)[0];
     uint64_t *list = (phase & 1) ? a.list1 : a.list0;
     uint64_t k = nch - 1;
     uint32_t base =
@@ -728,7 +728,7 @@ template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                             uint32_t phase, uint32_t n_items, int kind, uint64_t &edges)
 {
-    PhaseSlot *prev = &a.gs->slot[(phase - 1) & 1];
+    PhaseSlot *prev = &slot_of(a.gs, phase + 2)[0];
     const uint64_t *list = ((phase - 1) & 1) ? a.list1 : a.list0;
     if (n_items > a.list_cap)
         n_items = a.list_cap;
@@ -800,7 +800,7 @@ template <int BLOCK, int LQ, typename R, bool CNT>
 __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
                             uint32_t phase, uint64_t &edges)
 {
-    PhaseSlot *slot = &a.gs->slot[phase & 1];
+    PhaseSlot *slot = &slot_of(a.gs, phase)[0];
     uint64_t *list = (phase & 1) ? a.list1 : a.list0;
     const uint32_t nb = a.gs->nblocks;
     // Every access the protocol relies on is device-coherent (atomics, and
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             }
         }
         if (threadIdx.x == 0) {
-            PhaseSlot *s = &gs->slot[phase & 1];
+            PhaseSlot *s = &slot_of(gs, phase)[0];
             if (L.claims)
                 atomicAdd((unsigned long long *)&s->sum0, (unsigned long long)L.claims);
             if (L.minv != 0xffffffffu)
@@ -1101,7 +1101,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         }
         if (!grid_barrier<false>(gs, gen))
             break;
-        PhaseSlot *s = &gs->slot[phase & 1];
+        PhaseSlot *s = &slot_of(gs, phase)[0];
         uint64_t claims = ld_relaxed64(&s->sum0);
         uint32_t n_next = a.mode == MODE_GPP ? ld_relaxed(&s->qtail)
                                              : (uint32_t)(ld_relaxed64(&s->qword) >> 32);
@@ -1217,19 +1217,7 @@ __global__ void peel_init_kernel(const R *rowptr, uint32_t V, uint32_t *deg, uin
     for (uint32_t w = i; w < ndone; w += stride)
         done[w] = (w == ndone - 1 && (V & 63)) ? ~((1ull << (V & 63)) - 1) : 0ull;
     if (i == 0) {
-        gs->arrive = 0;
-        gs->gen = 0;
-        gs->err = 0;
-        gs->nblocks = nblocks;
-        gs->timeout_ticks = timeout;
-        gs->t0 = 0;
-        gs->live_next = 0;
-        for (int p = 0; p < 2; p++) {
-            gs->slot[p].sum0 = gs->slot[p].sum1 = gs->slot[p].sum2 = 0;
-            gs->slot[p].min0 = 0xffffffffu;
-            gs->slot[p].qtail = gs->slot[p].qhead = gs->slot[p].flag = 0;
-            gs->slot[p].qword = nblocks;
-        }
+        grid_sync_init(gs, nblocks, timeout);
         for (int k = 0; k < KC_ST_COUNT; k++)
             stats[k] = 0;
     }
