@@ -106,21 +106,23 @@ __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 // Each slot, the arrival counter and the generation word sit on cache lines
 // of their own (128 bytes), so that polling one does not contend with
 // atomics on another.
+// The counters a phase hits continuously (queue word, list tail, list head)
+// also get a line each.
 struct alignas(128) PhaseSlot {
     uint64_t sum0;  // engine-defined sums
     uint64_t sum1;
     uint64_t sum2;
     uint32_t min0;  // engine-defined minimum (cleared to UINT32_MAX)
-    uint32_t qtail; // items pushed to the shared list of this phase
-    uint32_t qhead; // grab counter for reading that list
     uint32_t flag;  // engine-defined OR
     uint32_t pad0;
     uint32_t pad1;
+    alignas(128) uint32_t qtail; // items pushed to the shared list of this phase
+    alignas(128) uint32_t qhead; // grab counter for reading that list
     // Work list of the peel's bucket / PeelOne modes: items pushed (high 32
     // bits) and pending work (low 32: workgroups still in their own work,
     // which starts at nblocks, plus pushed items not yet fully processed).
     // One word, so a single atomic load tells whether the phase is over.
-    uint64_t qword;
+    alignas(128) uint64_t qword;
 };
 
 struct GridSync {
