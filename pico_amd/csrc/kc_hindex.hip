@@ -178,7 +178,9 @@ __device__ __forceinline__ uint64_t wave_count_gt(const uint32_t *nd, uint64_t n
 }
 
 // Histogram of min(cap, val(u)) over u in N(v) into bins[0..cap].  LDS or
-// HBM bins; the caller fences afterwards.
+// HBM bins; the caller fences afterwards.  Values at or above cap (in the
+// dense core, most of a hub's neighbours) are counted in a register and
+// added to the top bin once, instead of one same-address atomic each.
 template <typename R, bool LDSB>
 __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_t cap,
                                           const uint32_t *val, uint32_t *bins)
@@ -188,6 +190,7 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
         bins[j] = 0;
     __threadfence_block();
     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+    uint32_t ntop = 0;
     for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
         uint32_t x[4];
 #pragma unroll
@@ -196,10 +199,18 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
             x[k] = ii < e ? val[a.col[ii]] : 0xffffffffu;
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (x[k] != 0xffffffffu)
-                atomicAdd(&bins[x[k] < cap ? x[k] : cap], 1u);
+        for (int k = 0; k < 4; k++) {
+            if (x[k] == 0xffffffffu)
+                continue;
+            if (x[k] >= cap)
+                ntop++;
+            else
+                atomicAdd(&bins[x[k]], 1u);
+        }
     }
+    ntop = wave_sum_u32(ntop);
+    if (lane == 0 && ntop)
+        atomicAdd(&bins[cap], ntop);
     __threadfence_block();
 }
 
@@ -319,10 +330,27 @@ __device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t
     vm_drain();
     __syncthreads();
     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-    for (uint64_t i = s + threadIdx.x; i < e; i += BLOCK) {
-        uint32_t x = a.core[a.col[i]];
-        atomicAdd(&bins[x < cap ? x : cap], 1u);
+    uint32_t ntop = 0; // values >= cap: one top-bin addition per wave (see wave_hist)
+    for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t ii = i + (uint64_t)k * BLOCK;
+            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (x[k] == 0xffffffffu)
+                continue;
+            if (x[k] >= cap)
+                ntop++;
+            else
+                atomicAdd(&bins[x[k]], 1u);
+        }
     }
+    ntop = wave_sum_u32(ntop);
+    if (lane_id() == 0 && ntop)
+        atomicAdd(&bins[cap], ntop);
     vm_drain();
     __syncthreads();
 }
