@@ -792,36 +792,56 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     if (lane == 0)
                         a.core[v] = a.newc[v];
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-                    for (uint64_t i0 = s; i0 < e; i0 += KC_WAVE) {
-                        uint64_t i = i0 + lane;
-                        bool fresh = false;
-                        uint32_t u = 0;
-                        if (i < e) {
-                            u = a.col[i];
-                            fresh = atomicMax(&a.mark[u], epoch) != epoch;
-                            n_atom++;
-                            if (a.method == HM_CNT)
-                                a.cnt[u] = 0;
+                    for (uint64_t i0 = s; i0 < e; i0 += 4 * KC_WAVE) {
+                        uint32_t u[4];
+                        bool fresh[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            uint64_t i = i0 + (uint64_t)k * KC_WAVE + lane;
+                            u[k] = i < e ? a.col[i] : 0xffffffffu;
                         }
-                        hpush(gs, phase, m_list, cap_list, fresh, u);
+                        // a mark read at this epoch is final (marks only
+                        // grow): most neighbours of hubs skip the atomic
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            fresh[k] = false;
+                            if (u[k] != 0xffffffffu && a.mark[u[k]] != epoch) {
+                                fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
+                                n_atom++;
+                                if (fresh[k] && a.method == HM_CNT)
+                                    a.cnt[u[k]] = 0;
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            hpush(gs, phase, m_list, cap_list, fresh[k], u[k]);
                     }
                 },
                 [&](uint32_t v) {
                     if (threadIdx.x == 0)
                         a.core[v] = a.newc[v];
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-                    for (uint64_t i0 = s; i0 < e; i0 += BLOCK) {
-                        uint64_t i = i0 + threadIdx.x;
-                        bool fresh = false;
-                        uint32_t u = 0;
-                        if (i < e) {
-                            u = a.col[i];
-                            fresh = atomicMax(&a.mark[u], epoch) != epoch;
-                            n_atom++;
-                            if (a.method == HM_CNT)
-                                a.cnt[u] = 0;
+                    for (uint64_t i0 = s; i0 < e; i0 += 4 * (uint64_t)BLOCK) {
+                        uint32_t u[4];
+                        bool fresh[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            uint64_t i = i0 + (uint64_t)k * BLOCK + threadIdx.x;
+                            u[k] = i < e ? a.col[i] : 0xffffffffu;
                         }
-                        hpush(gs, phase, m_list, cap_list, fresh, u);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            fresh[k] = false;
+                            if (u[k] != 0xffffffffu && a.mark[u[k]] != epoch) {
+                                fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
+                                n_atom++;
+                                if (fresh[k] && a.method == HM_CNT)
+                                    a.cnt[u[k]] = 0;
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            hpush(gs, phase, m_list, cap_list, fresh[k], u[k]);
                     }
                     __syncthreads();
                 });
@@ -849,8 +869,17 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     uint32_t cu = a.core[u];
                     uint32_t c = 0;
                     uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
-                    for (uint64_t i = s + lane; i < e; i += KC_WAVE)
-                        c += a.core[a.col[i]] >= cu;
+                    for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
+                        uint32_t w[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            uint64_t ii = i + (uint64_t)k * KC_WAVE;
+                            w[k] = ii < e ? a.col[ii] : 0xffffffffu;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            c += w[k] != 0xffffffffu && a.core[w[k]] >= cu;
+                    }
                     c = wave_sum_u32(c);
                     if (lane == 0)
                         a.cnt[u] = c;
@@ -860,8 +889,17 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     uint32_t cu = a.core[u];
                     uint32_t c = 0;
                     uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
-                    for (uint64_t i = s + threadIdx.x; i < e; i += BLOCK)
-                        c += a.core[a.col[i]] >= cu;
+                    for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
+                        uint32_t w[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            uint64_t ii = i + (uint64_t)k * BLOCK;
+                            w[k] = ii < e ? a.col[ii] : 0xffffffffu;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            c += w[k] != 0xffffffffu && a.core[w[k]] >= cu;
+                    }
                     c = block_sum<BLOCK>(L, c);
                     if (threadIdx.x == 0)
                         a.cnt[u] = c;
