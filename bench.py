@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only", choices=["peel", "batch", "serve"], default=None)
+    ap.add_argument("--pmc-stages", type=int, default=1,
+                    help="with --summarize-pmc: peel_kernel launches per plain-peel step "
+                         "(launches_per_step of the profiled run)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="write profiles/pmc_traffic.json from two rocprofv3 --pmc "
                          "databases (FETCH_SIZE, WRITE_SIZE) of this script's runs")
@@ -89,7 +92,8 @@ def traffic_from_profile(kernel_key):
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(kernel_key, {}).get("bytes_per_launch")
+            e = json.load(f).get(kernel_key, {})
+            return e.get("bytes_per_step", e.get("bytes_per_launch"))
     except (OSError, ValueError):
         return None
 
@@ -291,15 +295,19 @@ def _pmc_per_kernel(db_path, counter):
     return {k: sum(v) / len(v) for k, v in out.items()}
 
 
-def summarize_pmc(paths):
+def summarize_pmc(paths, peel_stages=1):
     """profiles/pmc_traffic.json: HBM bytes per launch of the dominant
     kernels = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled so that the
-    copy probe's counters read the bytes it is known to move."""
+    copy probe's counters read the bytes it is known to move.  One plain-peel
+    step is `peel_stages` launches of peel_kernel (one per compacted stage);
+    its bytes per step are the mean per launch times that count."""
     fetch = _pmc_per_kernel(paths[0], "FETCH_SIZE")
     write = _pmc_per_kernel(paths[1], "WRITE_SIZE")
 
     def pick(d, key):
-        ks = [k for k in d if key in k]
+        # the counting variant of peel_kernel (template argument true) is
+        # run once outside the timed steps: not the kernel measured
+        ks = [k for k in d if key in k and "true>" not in k]
         return d[ks[0]] if ks else None
 
     cf, cw = pick(fetch, "copy_probe_kernel"), pick(write, "copy_probe_kernel")
@@ -313,8 +321,11 @@ def summarize_pmc(paths):
         f, w = pick(fetch, kname), pick(write, kname)
         if f is None or w is None:
             continue
+        per = int(f * 1024.0 * sf + w * 1024.0 * sw)
+        n = peel_stages if key == "peel" else 1
         out[key] = {"kernel": kname, "fetch_kib": f, "write_kib": w,
-                    "bytes_per_launch": int(f * 1024.0 * sf + w * 1024.0 * sw)}
+                    "bytes_per_launch": per, "launches_per_step": n,
+                    "bytes_per_step": per * n}
     with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
@@ -332,7 +343,7 @@ def main():
     import pico_amd as pa
 
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc)
+        summarize_pmc(args.summarize_pmc, args.pmc_stages)
         return
     if args.only == "peel":
         args.no_batch = args.no_serve = True
