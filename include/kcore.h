@@ -87,7 +87,9 @@ typedef struct kc_opts {
                              at least 4,096), continue on a compacted copy of the live
                              subgraph, again at each further 1/compact_div; 0: default (8),
                              0xffffffff: never */
-    uint32_t reserved[2];
+    uint32_t stage_grid; /* bucket peels: persistent workgroups for the compacted stages
+                            (<= occupancy * CUs); 0: the grid of the first stage */
+    uint32_t reserved[1];
 } kc_opts;
 
 typedef struct kc_stats {

@@ -910,12 +910,21 @@ kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *bes
     for (const kc_opts &o : cand)
         if (kc_status s = trial(o))
             return s;
-    // the bucket peels' compaction ratio, with the best of the rest
+    // the bucket peels' compaction ratio and the compacted stages' grid,
+    // with the best of the rest
     if (method == KC_PEEL_DYN || method == KC_PEELONE_DYN) {
-        const kc_opts base = bo;
+        kc_opts base = bo;
         for (uint32_t cd : {4u, 16u, 0xffffffffu}) {
             kc_opts o = base;
             o.compact_div = cd;
+            if (kc_status s = trial(o))
+                return s;
+        }
+        base = bo;
+        const uint32_t ncu = g->ctx->num_cus;
+        for (uint32_t sg : {ncu / 4, ncu / 2, ncu}) {
+            kc_opts o = base;
+            o.stage_grid = std::max(1u, sg);
             if (kc_status s = trial(o))
                 return s;
         }

@@ -464,7 +464,7 @@ __global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
 static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     kc_ctx *c = g->ctx;
-    uint32_t block = o.block ? o.block : 512;
+    uint32_t block = o.block ? o.block : 1024;
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
     int occ = kc::peel_occupancy(block, g->wide);
@@ -634,12 +634,19 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         p.V = Vn;
         p.deg = (uint32_t *)g->cs_deg[nxt].p;
         p.done = (uint64_t *)g->cs_done[nxt].p;
-        uint64_t cap_n = (uint64_t)Vn + nnz_n / hub + 2ull * grid + 64;
+        uint64_t cap_n = (uint64_t)Vn + nnz_n / hub + 2ull * maxgrid + 64;
         p.list_cap = (uint32_t)std::min<uint64_t>(cap_n, cap);
         p.live_thresh = Vn / 4;
         p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : Vn / agg_div;
         p.tail_live = tail_of(Vn);
         p.start_level = (uint32_t)sv[kc::KC_ST_TAIL_LEVEL];
+        if (stage == 0) {
+            // the compacted stages are latency-bound: fewer workgroups make
+            // their barriers and shared-list polling cheaper (R-MAT s22 with
+            // 1024-thread groups: 3/8 of the CUs is the measured optimum)
+            uint32_t sg = o.stage_grid ? o.stage_grid : std::max(1u, maxgrid * 3 / 8);
+            p.grid = std::max(1u, std::min(sg, maxgrid));
+        }
         if (p.trace) {
             uint64_t used = std::min<uint64_t>(acc_phases, KC_TRACE_CAP);
             p.trace = (uint64_t *)g->trace.p + used * kc::KC_TRACE_WORDS;

@@ -82,6 +82,10 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
         o.agg_div = t.agg_div;
     if (!o.compact_div)
         o.compact_div = t.compact_div;
+    if (!o.grid)
+        o.grid = t.grid;
+    if (!o.stage_grid)
+        o.stage_grid = t.stage_grid;
     return o;
 }
 

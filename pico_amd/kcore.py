@@ -38,7 +38,7 @@ class Opts(ctypes.Structure):
                 ("gpp_compat", ctypes.c_uint32), ("part_size", ctypes.c_uint32),
                 ("local_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("agg_div", ctypes.c_uint32), ("compact_div", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("stage_grid", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
 class Stats(ctypes.Structure):
@@ -515,7 +515,8 @@ class DeviceGraph:
         ms = ctypes.c_double(0)
         _check(lib().kc_autotune(self._h, m, reps, ctypes.byref(o), ctypes.byref(ms)))
         return {"block": o.block, "hub_split": o.hub_split, "local_cap": o.local_cap,
-                "agg_div": o.agg_div, "compact_div": o.compact_div}, ms.value
+                "agg_div": o.agg_div, "compact_div": o.compact_div,
+                "grid": o.grid, "stage_grid": o.stage_grid}, ms.value
 
     def __init__(self, ctx, handle, V):
         self.ctx = ctx

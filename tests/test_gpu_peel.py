@@ -182,6 +182,9 @@ def test_compacted_stages(ctx, method, div, gen):
         for block in (256, 1024):
             again, _ = dg.decompose(method, compact_div=div, block=block)
             np.testing.assert_array_equal(again, core)
+        for sg in (1, 8, 1000):  # compacted stages on 1, 8 or all workgroups
+            again, _ = dg.decompose(method, compact_div=div, stage_grid=sg)
+            np.testing.assert_array_equal(again, core)
     finally:
         dg.close()
     want = oracle.bz(hg.rowptr, hg.col)
