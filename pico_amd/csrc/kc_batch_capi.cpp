@@ -885,9 +885,12 @@ kc_status kc_autotune(kc_graph *g, kc_method method, uint32_t reps, kc_opts *bes
                         cand.push_back(o);
                     }
         } else {
-            kc_opts o{};
-            o.block = block;
-            cand.push_back(o);
+            for (uint32_t gdiv : {0u, 2u}) { // all CUs' worth, or half
+                kc_opts o{};
+                o.block = block;
+                o.grid = gdiv ? std::max(1u, g->ctx->num_cus / gdiv) : 0u;
+                cand.push_back(o);
+            }
         }
     }
     kc_opts bo{};

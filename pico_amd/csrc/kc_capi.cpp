@@ -476,7 +476,13 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     // observed not to be co-resident).
     uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
     uint32_t maxgrid = per_cu * c->num_cus;
-    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : maxgrid;
+    // Default widths measured on R-MAT s22: the level-synchronous GPP and the
+    // PeelOne sweeps are latency-bound and run best on 1/2 and 3/8 of the
+    // CUs; the bucket peel's first stage uses them all.
+    uint32_t defgrid = method == KC_GPP      ? std::max(1u, maxgrid / 2)
+                       : method == KC_PEELONE ? std::max(1u, maxgrid * 3 / 8)
+                                              : maxgrid;
+    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : defgrid;
     uint32_t hub = o.hub_split ? o.hub_split : 1024;
     if (hub < 64 || hub > (1u << 24))
         return fail(KC_EINVAL, "hub_split must be in [64, 2^24]");
@@ -689,9 +695,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
 static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     kc_ctx *c = g->ctx;
-    // HistoCore's update phase favours more, smaller workgroups (each with
-    // its own hub aggregation table); the others favour 512
-    uint32_t block = o.block ? o.block : (method == KC_HISTO ? 256 : 512);
+    uint32_t block = o.block ? o.block : 512;
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
     int occ = kc::hindex_occupancy(block, g->wide);
@@ -699,7 +703,10 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         return fail(KC_EDEVICE, "h-index kernel cannot be resident with block %u", block);
     uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
     uint32_t maxgrid = per_cu * c->num_cus;
-    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : maxgrid;
+    // HistoCore runs best with one workgroup per CU, CntCore / NbrCore with
+    // as many as fit (R-MAT s22)
+    uint32_t defgrid = method == KC_HISTO ? std::min(maxgrid, c->num_cus) : maxgrid;
+    uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : defgrid;
     size_t V4 = (size_t)(g->V ? g->V : 1) * 4;
     kc_status s;
     if ((s = ensure(g, g->deg, V4, "core")) ||
