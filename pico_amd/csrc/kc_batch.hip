@@ -24,7 +24,7 @@
 
 namespace kc {
 
-static constexpr int BU = 4; // cascade edges per thread per pass
+static constexpr int BU = 8; // cascade edges per thread per pass
 
 template <int BLOCK, int LDS_V> struct BLds {
     uint32_t deg[LDS_V];
@@ -158,30 +158,42 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchArgs a)
             __syncthreads();
             const uint32_t nl = L.nlive;
             uint32_t lmin = 0xffffffffu;
-            for (uint32_t b0 = 0; b0 < nl; b0 += BLOCK) {
-                uint32_t i = b0 + threadIdx.x;
-                uint32_t v = i < nl ? live[i] : 0u;
+            constexpr uint32_t SU = 4; // live entries per thread per step, loads in flight
+            for (uint32_t b0 = 0; b0 < nl; b0 += BLOCK * SU) {
+                uint32_t v[SU], d[SU];
+#pragma unroll
+                for (uint32_t k = 0; k < SU; k++) {
+                    uint32_t i = b0 + k * BLOCK + threadIdx.x;
+                    v[k] = i < nl ? live[i] : 0xffffffffu;
+                }
                 // exact values: the scan never overlaps the cascade, and HBM
                 // degrees are read past the L1 (the cascade's atomics run in L2)
-                uint32_t d = i < nl ? (in_lds ? deg[v] : ld_relaxed(&deg[v])) : 0xffffffffu;
-                bool hit = i < nl && d == level;
-                bool keep = i < nl && d > level;
-                if (keep)
-                    lmin = d < lmin ? d : lmin;
-                __syncthreads(); // every read of this block done before writes
-                b_push<BLOCK, LDS_V>(L, q, hit, v);
-                uint64_t km = ballot(keep);
-                if (km) {
-                    uint32_t leader = __ffsll((unsigned long long)km) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader)
-                        base = atomicAdd(&L.nkeep, (uint32_t)__popcll(km));
-                    base = __shfl(base, leader, KC_WAVE);
+#pragma unroll
+                for (uint32_t k = 0; k < SU; k++)
+                    d[k] = v[k] != 0xffffffffu ? (in_lds ? deg[v[k]] : ld_relaxed(&deg[v[k]]))
+                                               : 0xffffffffu;
+                __syncthreads(); // every read of this step done before writes
+#pragma unroll
+                for (uint32_t k = 0; k < SU; k++) {
+                    bool hit = v[k] != 0xffffffffu && d[k] == level;
+                    bool keep = v[k] != 0xffffffffu && d[k] > level;
                     if (keep)
-                        live[base + lane_rank(km)] = v; // base <= b0: already read
+                        lmin = d[k] < lmin ? d[k] : lmin;
+                    b_push<BLOCK, LDS_V>(L, q, hit, v[k]);
+                    uint64_t km = ballot(keep);
+                    if (km) {
+                        uint32_t leader = __ffsll((unsigned long long)km) - 1;
+                        uint32_t base = 0;
+                        if (lane == leader)
+                            base = atomicAdd(&L.nkeep, (uint32_t)__popcll(km));
+                        base = __shfl(base, leader, KC_WAVE);
+                        // kept entries <= entries read so far: positions < b0 + BLOCK*SU
+                        if (keep)
+                            live[base + lane_rank(km)] = v[k];
+                    }
                 }
-                __syncthreads();
             }
+            __syncthreads();
             uint32_t mn = block_min_u32<BLOCK>(lmin, &L.minv);
             if (threadIdx.x == 0)
                 L.nlive = L.nkeep;
