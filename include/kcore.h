@@ -160,6 +160,20 @@ KC_API kc_status kc_hg_vgraph(const kc_host_graph *g, uint32_t part_size,
 KC_API kc_status kc_hg_write_edgelist(const kc_host_graph *g, const char *path);
 KC_API void kc_hg_free(kc_host_graph *g);
 
+/* Binary CSR files and a content-keyed reader cache, in place of the text
+ * ./output/serialized-<basename> files of Graph::readSerialized /
+ * writeSerialized (peel/src/graph.cpp:9-54).  A cache entry records the
+ * 64-bit content hash and size of the edge file and the reader that parsed
+ * it, and is named after that hash: lookups are by content, not by name.  A
+ * damaged, truncated or mismatched entry counts as a miss: the edge file is
+ * parsed again and the entry rewritten. */
+#define KC_READER_COO 0   /* kc_hg_read_coo */
+#define KC_READER_DEDUP 1 /* kc_hg_read_dedup */
+KC_API kc_status kc_hg_save(const kc_host_graph *g, const char *path);
+KC_API kc_status kc_hg_load(const char *path, kc_host_graph **out);
+KC_API kc_status kc_hg_read_cached(const char *path, int reader, const char *cache_dir,
+                                   kc_host_graph **out, int *hit);
+
 /* "i core\n" per vertex, the format of utility::SaveResults (util.cpp:70-89) */
 KC_API kc_status kc_write_cores(const char *path, const uint32_t *core, uint32_t V);
 

@@ -39,6 +39,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "kc_dyn_capi.cpp"),
     os.path.join(CSRC, "kc_server.cpp"),
     os.path.join(HOST, "kc_host.cpp"),
+    os.path.join(HOST, "kc_cache.cpp"),
 ]
 
 CLI_TOOLS = ["gpp", "peelone", "peelone_dyn", "HistoCore", "CntCore", "NbrCore"]

@@ -85,6 +85,76 @@ kc_status kc_hg_read_coo(const char *path, kc_host_graph **out)
     return KC_OK;
 }
 
+kc_status kc_hg_save(const kc_host_graph *g, const char *path)
+{
+    if (!g || !path)
+        return fail(KC_EINVAL, "null argument");
+    std::string err;
+    if (kch::save_csr(*g, path, kch::KC_NO_READER, 0, 0, err))
+        return fail(KC_EIO, "%s", err.c_str());
+    return KC_OK;
+}
+
+kc_status kc_hg_load(const char *path, kc_host_graph **out)
+{
+    if (!path || !out)
+        return fail(KC_EINVAL, "null argument");
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    std::string err;
+    if (kch::load_csr(path, *g, nullptr, nullptr, nullptr, err)) {
+        delete g;
+        return fail(KC_EIO, "%s", err.c_str());
+    }
+    *out = g;
+    return KC_OK;
+}
+
+kc_status kc_hg_read_cached(const char *path, int reader, const char *cache_dir,
+                            kc_host_graph **out, int *hit)
+{
+    if (!path || !out || !cache_dir)
+        return fail(KC_EINVAL, "null argument");
+    if (reader != KC_READER_COO && reader != KC_READER_DEDUP)
+        return fail(KC_EINVAL, "unknown reader %d", reader);
+    if (hit)
+        *hit = 0;
+    std::string err;
+    uint64_t hash = 0, size = 0;
+    if (kch::file_hash(path, hash, size, err))
+        return fail(KC_EIO, "%s", err.c_str());
+    char name[64];
+    std::snprintf(name, sizeof name, "/kcsr-%016llx-%d.bin", (unsigned long long)hash, reader);
+    std::string entry = std::string(cache_dir) + name;
+    auto *g = new (std::nothrow) kc_host_graph;
+    if (!g)
+        return fail(KC_ENOMEM, "host allocation");
+    uint32_t r = 0;
+    uint64_t h2 = 0, s2 = 0;
+    if (kch::load_csr(entry.c_str(), *g, &r, &h2, &s2, err) == 0 && r == (uint32_t)reader &&
+        h2 == hash && s2 == size) {
+        if (hit)
+            *hit = 1;
+        *out = g;
+        return KC_OK;
+    }
+    // miss (absent, stale or damaged entry): parse, then (re)write the entry
+    *g = kc_host_graph{};
+    int rc = reader == KC_READER_COO ? kch::read_coo(path, *g, err)
+                                     : kch::read_dedup(path, *g, err);
+    if (rc) {
+        delete g;
+        return fail(err.rfind("cannot", 0) == 0 ? KC_EIO : KC_EINVAL, "%s", err.c_str());
+    }
+    if (kch::save_csr(*g, entry.c_str(), (uint32_t)reader, hash, size, err)) {
+        delete g;
+        return fail(KC_EIO, "%s", err.c_str());
+    }
+    *out = g;
+    return KC_OK;
+}
+
 kc_status kc_hg_read_dedup(const char *path, kc_host_graph **out)
 {
     if (!path || !out)

@@ -11,6 +11,9 @@
 //     repetitions of peel_dyn and peelone_dyn, drop min and max).
 // KC_SAVE_RESULTS=<path> additionally writes "i core" lines in the format of
 // utility::SaveResults (util.cpp:70-89), which the reference never reaches.
+// KC_CSR_CACHE=<dir> reads the edge file through the content-keyed binary
+// CSR cache in <dir> (the reference's ./output/serialized-<name> text cache,
+// peel/src/graph.cpp:9-54); stdout is the same either way.
 #include "kcore.h"
 
 #include <algorithm>
@@ -50,6 +53,14 @@ static void print_max_min(const std::vector<uint32_t> &core)
     std::printf(" Max K = %u\n Min K = %u\n", mx, mn);
 }
 
+static kc_status read_graph(const char *path, int reader, kc_host_graph **out)
+{
+    const char *dir = std::getenv("KC_CSR_CACHE");
+    if (dir && *dir)
+        return kc_hg_read_cached(path, reader, dir, out, nullptr);
+    return reader == KC_READER_COO ? kc_hg_read_coo(path, out) : kc_hg_read_dedup(path, out);
+}
+
 static int die(const char *what)
 {
     std::fprintf(stderr, "%s: %s\n", what, kc_last_error());
@@ -77,7 +88,7 @@ static int run_vgraph_program(const std::string &prog, const char *path)
                                         : KC_NBR;
     std::printf("Reading the input graph from the following file:\n>> %s\n", path);
     kc_host_graph *hg = nullptr;
-    if (kc_hg_read_coo(path, &hg) != KC_OK)
+    if (read_graph(path, KC_READER_COO, &hg) != KC_OK)
         return die("read");
     kc_graph_info info;
     kc_hg_info(hg, &info);
@@ -137,7 +148,7 @@ static int run_peelone_dyn(int argc, char **argv)
     }
     std::printf("Graph loading Started... \n");
     kc_host_graph *hg = nullptr;
-    if (kc_hg_read_dedup(argv[1], &hg) != KC_OK)
+    if (read_graph(argv[1], KC_READER_DEDUP, &hg) != KC_OK)
         return die("read");
     kc_graph_info info;
     kc_hg_info(hg, &info);

@@ -79,6 +79,16 @@ int csr_from_pairs(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *
                    kc_host_graph &g);
 
 int gen_er(uint32_t n, uint64_t m, uint64_t seed, kc_host_graph &g);
+
+// Binary CSR files (kc_cache.cpp).  `reader` records which reader produced
+// the graph (KC_NO_READER for none); src_hash / src_size identify the edge
+// file it was read from.
+constexpr uint32_t KC_NO_READER = 0xffffffffu;
+int file_hash(const char *path, uint64_t &hash, uint64_t &size, std::string &err);
+int save_csr(const kc_host_graph &g, const char *path, uint32_t reader, uint64_t src_hash,
+             uint64_t src_size, std::string &err);
+int load_csr(const char *path, kc_host_graph &g, uint32_t *reader, uint64_t *src_hash,
+             uint64_t *src_size, std::string &err);
 int gen_rmat(uint32_t scale, uint64_t m_gen, double a, double b, double c, uint64_t seed,
              kc_host_graph &g);
 int gen_chunglu(uint32_t n, double beta, double mean_deg, double cap, uint64_t seed,

@@ -174,6 +174,10 @@ def lib():
         "kc_last_error": ([], ctypes.c_char_p),
         "kc_version": ([], ctypes.c_char_p),
         "kc_hg_read_coo": ([ctypes.c_char_p, ctypes.POINTER(_vp)], S),
+        "kc_hg_save": ([_vp, ctypes.c_char_p], S),
+        "kc_hg_load": ([ctypes.c_char_p, ctypes.POINTER(_vp)], S),
+        "kc_hg_read_cached": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                               ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int)], S),
         "kc_hg_read_dedup": ([ctypes.c_char_p, ctypes.POINTER(_vp)], S),
         "kc_hg_from_pairs": ([ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, ctypes.POINTER(_vp)], S),
         "kc_hg_from_csr": ([ctypes.c_uint32, _vp, _vp, ctypes.POINTER(_vp)], S),
@@ -346,6 +350,25 @@ class HostGraph:
     @classmethod
     def read_dedup(cls, path):
         return cls._make(lib().kc_hg_read_dedup, os.fsencode(path))
+
+    @classmethod
+    def load(cls, path):
+        """A graph written by save() (binary CSR)."""
+        return cls._make(lib().kc_hg_load, os.fsencode(path))
+
+    def save(self, path):
+        _check(lib().kc_hg_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def read_cached(cls, path, reader, cache_dir):
+        """Read `path` with reader "coo" or "dedup" through the content-keyed
+        binary cache in `cache_dir`; returns (graph, hit)."""
+        r = {"coo": KC_READER_COO, "dedup": KC_READER_DEDUP}[reader]
+        h = _vp()
+        hit = ctypes.c_int(0)
+        _check(lib().kc_hg_read_cached(os.fsencode(path), r, os.fsencode(cache_dir),
+                                       ctypes.byref(h), ctypes.byref(hit)))
+        return cls(h), bool(hit.value)
 
     @classmethod
     def from_pairs(cls, V, src, dst):
@@ -790,6 +813,8 @@ def _trace_sig():
 KC_OPT_TRACE = 1
 KC_OPT_NO_DEGSORT = 2
 KC_OPT_COUNT = 4
+KC_READER_COO = 0
+KC_READER_DEDUP = 1
 
 
 def device_trace(dgraph, max_phases=1 << 16):

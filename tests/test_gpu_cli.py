@@ -88,3 +88,25 @@ def test_peelone_dyn_program(tmp_path):
     assert re.search(r"Our_Paper_peel: (\d+ms ){10} ave: [0-9.e+-]+", out), out
     assert [int(x) for x in re.findall(r"Kmax: (\d+)", out)] == [2, 2]
     assert [int(x) for x in save.read_text().split()[1::2]] == [2] * 5
+
+
+@pytest.mark.parametrize("name", ["peelone_dyn", "HistoCore"])
+def test_programs_through_csr_cache(name, tmp_path):
+    # KC_CSR_CACHE: the first run parses and writes the entry, the second
+    # loads it; stdout lines and the saved cores are the same as without it
+    hg = pa.HostGraph.rmat(11, 8, seed=7)
+    f = tmp_path / "g.txt"
+    hg.write_edgelist(str(f))
+    cache = tmp_path / "cache"
+    cache.mkdir()
+    outs, cores = [], []
+    for env in ({}, {"KC_CSR_CACHE": str(cache)}, {"KC_CSR_CACHE": str(cache)}):
+        save = tmp_path / "cores.txt"
+        env = dict(env, KC_SAVE_RESULTS=str(save))
+        out = _run(name, str(f), env)
+        outs.append(re.sub(r"[0-9.e+-]+ ?\(?ms\)?|\d+ms |ave: [0-9.e+-]+", "", out))
+        cores.append(save.read_text())
+        if env.get("KC_CSR_CACHE"):
+            assert len(os.listdir(cache)) == 1
+    assert outs[0] == outs[1] == outs[2]
+    assert cores[0] == cores[1] == cores[2]
