@@ -181,6 +181,18 @@ KC_API kc_status kc_write_cores(const char *path, const uint32_t *core, uint32_t
 typedef struct kc_ctx kc_ctx;
 typedef struct kc_graph kc_graph;
 
+/* Degree statistics of a resident graph: what peelone_dyn prints before it
+ * peels (main.cu:10-63: PrintMax's " Maxdegree = ", STDdegrees' population
+ * standard deviation). */
+typedef struct kc_degree_stats {
+    uint32_t V;
+    uint32_t dmax;
+    uint64_t nnz;
+    double mean;
+    double stddev; /* population standard deviation of the degrees */
+    uint64_t reserved[2];
+} kc_degree_stats;
+
 KC_API kc_status kc_ctx_create(int device, kc_ctx **out);
 KC_API void kc_ctx_destroy(kc_ctx *ctx);
 KC_API kc_status kc_ctx_info(const kc_ctx *ctx, uint32_t *num_cus, uint64_t *hbm_bytes);
@@ -381,6 +393,16 @@ KC_API kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u, cons
                            uint64_t n, uint32_t *core_out, kc_update_stats *stats);
 /* current vertex count and adjacency entries (2x undirected edges) */
 KC_API kc_status kc_graph_dims(const kc_graph *g, uint32_t *V, uint64_t *nnz);
+KC_API kc_status kc_graph_degree_stats(kc_graph *g, kc_degree_stats *out);
+/* The k-core of a resident graph as a new resident graph on the same
+ * context: the vertices whose core number from the last decomposition of g
+ * (kc_decompose, or the cores kc_update maintains) is >= k, renumbered in id
+ * order, with the edges among them in their original row order.  k = kmax
+ * gives the kmax-core.  ids (room for g's V entries, may be NULL) receives
+ * the id in g of each new vertex; *n (may be NULL) the new vertex count.
+ * KC_EINVAL if g has not been decomposed. */
+KC_API kc_status kc_kcore_subgraph(kc_graph *g, uint32_t k, kc_graph **out, uint32_t *ids,
+                                   uint32_t *n);
 
 /* ------------------------------------------------------------- serving */
 /* A request loop on one GPU (no counterpart in the reference).  Worker

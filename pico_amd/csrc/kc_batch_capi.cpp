@@ -4,6 +4,7 @@
 #include "kc_capi_impl.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstring>
 #include <new>
@@ -489,6 +490,128 @@ kc_status kc_graph_download(const kc_graph *g, kc_host_graph **out)
     }
     *out = h;
     return KC_OK;
+}
+
+kc_status kc_graph_degree_stats(kc_graph *g, kc_degree_stats *out)
+{
+    if (!g || !out)
+        return fail(KC_EINVAL, "null argument");
+    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    if (g->csr_stale) {
+        kc_status s = dyn_materialize(g);
+        if (s)
+            return s;
+    }
+    hipStream_t st = g->ctx->stream;
+    if (kc_status s = ensure(g, g->aux, 64, "degree statistics"))
+        return s;
+    unsigned long long r[4] = {0, 0, 0, 0};
+    KC_HIP(hipMemsetAsync(g->aux.p, 0, 32, st), "memset");
+    KC_HIP(kc::degree_stats(g->rowptr, g->wide, g->V, (unsigned long long *)g->aux.p, st),
+           "degree statistics");
+    KC_HIP(hipMemcpyAsync(r, g->aux.p, 32, hipMemcpyDeviceToHost, st), "copy");
+    KC_HIP(hipStreamSynchronize(st), "degree statistics");
+    *out = kc_degree_stats{};
+    out->V = g->V;
+    out->dmax = (uint32_t)r[0];
+    out->nnz = r[1];
+    if (g->V) {
+        // population variance (V * sum d^2 - (sum d)^2) / V^2, exact up to the division
+        unsigned __int128 s2 = (unsigned __int128)r[2] + ((unsigned __int128)r[3] << 32);
+        unsigned __int128 num = s2 * g->V - (unsigned __int128)r[1] * r[1];
+        long double var = (long double)num / ((long double)g->V * (long double)g->V);
+        out->mean = (double)((long double)r[1] / g->V);
+        out->stddev = (double)std::sqrt(var);
+    }
+    return KC_OK;
+}
+
+kc_status kc_kcore_subgraph(kc_graph *g, uint32_t k, kc_graph **out, uint32_t *ids, uint32_t *n)
+{
+    if (!g || !out)
+        return fail(KC_EINVAL, "null argument");
+    if (!g->result)
+        return fail(KC_EINVAL, "the graph has no core numbers yet: decompose it first");
+    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    if (g->csr_stale) {
+        kc_status s = dyn_materialize(g);
+        if (s)
+            return s;
+    }
+    hipStream_t st = g->ctx->stream;
+    const uint32_t V = g->V, nw = (V + 63) / 64;
+    kc_status s;
+    if ((s = ensure(g, g->cs_done[0], (size_t)nw * 8 + 8, "subgraph")) ||
+        (s = ensure(g, g->cs_deg[0], (size_t)V * 4 + 4, "subgraph")) ||
+        (s = ensure(g, g->cs_wpre, (size_t)(nw + 1) * 4, "subgraph")) ||
+        (s = ensure(g, g->cs_bad, 8, "subgraph")))
+        return s;
+    uint64_t *excl = (uint64_t *)g->cs_done[0].p;
+    uint32_t *deg = (uint32_t *)g->cs_deg[0].p;
+    KC_HIP(kc::kcore_exclusion(g->result, V, k, excl, st), "k-core marks");
+    KC_HIP(kc::kcore_row_lengths(g->rowptr, g->wide, g->col, excl, V, deg, st), "k-core rows");
+    std::vector<uint64_t> words(nw);
+    if (nw)
+        KC_HIP(hipMemcpyAsync(words.data(), excl, (size_t)nw * 8, hipMemcpyDeviceToHost, st),
+               "copy");
+    KC_HIP(hipStreamSynchronize(st), "k-core marks");
+    uint32_t Vn = 0;
+    for (uint64_t w : words)
+        Vn += (uint32_t)__builtin_popcountll(~w);
+    const bool wide_n = g->nnz >= (1ull << 32);
+    const size_t rb = wide_n ? 8 : 4;
+    if ((s = ensure(g, g->cs_loc, (size_t)Vn * 4 + 4, "subgraph")) ||
+        (s = ensure(g, g->cs_gmap[0], (size_t)Vn * 4 + 4, "subgraph")) ||
+        (s = ensure(g, g->cs_tmp, kc::compact_tmp_bytes(V, Vn), "subgraph")))
+        return s;
+    void *rp = nullptr;
+    uint32_t *col = nullptr;
+    KC_HIP(hipMalloc(&rp, (Vn + 1ull) * rb), "hipMalloc subgraph rows");
+    KC_HIP(hipMemsetAsync(g->cs_bad.p, 0, 8, st), "memset");
+    kc::CompactIn in{g->rowptr, g->wide, g->col, V, deg, excl, nullptr};
+    kc::CompactOut co{};
+    co.V_in = V;
+    co.Vn = Vn;
+    co.wpre = (uint32_t *)g->cs_wpre.p;
+    co.loc = (uint32_t *)g->cs_loc.p;
+    co.gmap = (uint32_t *)g->cs_gmap[0].p;
+    co.rowptr = rp;
+    co.wide = wide_n;
+    co.bad = (uint32_t *)g->cs_bad.p;
+    co.tmp = g->cs_tmp.p;
+    co.tmp_bytes = g->cs_tmp.n;
+    uint64_t nnz_n = 0;
+    uint32_t bad[2] = {0, 0};
+    hipError_t e = kc::compact_rows(in, co, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&nnz_n, (char *)rp + (size_t)Vn * rb, rb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bad, co.bad, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e == hipSuccess && bad[0] == 0 && nnz_n <= g->nnz)
+        e = hipMalloc(&col, nnz_n ? nnz_n * 4 : 4);
+    if (e == hipSuccess && bad[0] == 0 && col) {
+        co.col = col;
+        co.col_cap = nnz_n;
+        e = kc::compact_cols(in, co, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(bad, co.bad, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && ids && Vn)
+            e = hipMemcpyAsync(ids, co.gmap, (size_t)Vn * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+    }
+    if (e != hipSuccess || bad[0] || !col) {
+        hipFree(rp);
+        hipFree(col);
+        if (e != hipSuccess)
+            return hip_fail(e, "k-core subgraph");
+        return fail(KC_EDEVICE, "k-core subgraph: row lengths do not match (flags %u)", bad[0]);
+    }
+    if (n)
+        *n = Vn;
+    return graph_adopt(g->ctx, Vn, nnz_n, wide_n, rp, col, out);
 }
 
 kc_status kc_batch_gen_rmat_list(kc_ctx *ctx, uint32_t G, const uint32_t *scale,

@@ -465,7 +465,7 @@ void kc_graph_free(kc_graph *g)
                       &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->cs_row[0], &g->cs_row[1],
                       &g->cs_col[0], &g->cs_col[1], &g->cs_deg[0], &g->cs_deg[1], &g->cs_done[0],
                       &g->cs_done[1], &g->cs_gmap[0], &g->cs_gmap[1], &g->cs_loc, &g->cs_wpre,
-                      &g->cs_tmp, &g->cs_bad})
+                      &g->cs_tmp, &g->cs_bad, &g->aux})
         if (b->p)
             hipFree(b->p);
     if (g->ev0)

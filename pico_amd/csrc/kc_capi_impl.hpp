@@ -57,6 +57,7 @@ struct kc_graph {
     DevBuf col_ds, nd_ds, hb_ds; // degree-ordered adjacency (built on first HistoCore run)
     // bucket peel on compacted live subgraphs: two alternating stage sets
     DevBuf cs_row[2], cs_col[2], cs_deg[2], cs_done[2], cs_gmap[2], cs_loc, cs_wpre, cs_tmp, cs_bad;
+    DevBuf aux; // small reductions (degree statistics)
     bool degsorted = false;
     uint32_t trace_phases = 0;
     const uint32_t *result = nullptr;

@@ -264,6 +264,80 @@ template <typename R2> size_t scan_bytes(uint32_t n)
     return b;
 }
 
+// k-core subgraph: excluded vertices (core < k, and the padding past V) are
+// the set bits of a claim-bitmap-shaped word array, so the compaction above
+// applies unchanged.
+__global__ void kcore_mark(const uint32_t *core, uint32_t V, uint32_t k, uint64_t *excl,
+                           uint32_t nw)
+{
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; // blockDim is a multiple of 64
+    uint64_t m = ballot(v >= V || core[v < V ? v : 0] < k);
+    if (lane_id() == 0 && v / 64 < nw)
+        excl[v / 64] = m;
+}
+
+// Row lengths of the subgraph: live neighbours of each live vertex (0 for
+// excluded ones), a wave per row with 4 x 64 entries in flight.
+template <typename R>
+__global__ __launch_bounds__(CB) void count_live(const R *rowptr, const uint32_t *col,
+                                                 const uint64_t *excl, uint32_t V, uint32_t *deg)
+{
+    constexpr uint32_t U = 4;
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (CB / KC_WAVE);
+    for (uint32_t v = blockIdx.x * (CB / KC_WAVE) + threadIdx.x / KC_WAVE; v < V; v += nwaves) {
+        if ((excl[v >> 6] >> (v & 63)) & 1ull) {
+            if (lane == 0)
+                deg[v] = 0;
+            continue;
+        }
+        uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
+        uint32_t c = 0;
+        for (uint64_t j0 = s; j0 < e; j0 += (uint64_t)KC_WAVE * U) {
+            uint32_t u[U];
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++) {
+                uint64_t j = j0 + (uint64_t)k * KC_WAVE + lane;
+                u[k] = j < e ? col[j] : 0xffffffffu;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < U; k++)
+                c += u[k] != 0xffffffffu && !((excl[u[k] >> 6] >> (u[k] & 63)) & 1ull);
+        }
+        c = wave_sum_u32(c);
+        if (lane == 0)
+            deg[v] = c;
+    }
+}
+
+// Degree statistics: out[0] max, out[1] sum, out[2] / out[3] the low / high
+// 32-bit halves of each squared degree, summed (exact to 2^64 per half).
+template <typename R>
+__global__ void degree_stats_kernel(const R *rowptr, uint32_t V, unsigned long long *out)
+{
+    uint32_t mx = 0;
+    uint64_t sum = 0, lo = 0, hi = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        uint64_t d = (uint64_t)(rowptr[v + 1] - rowptr[v]);
+        uint32_t d32 = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
+        mx = d32 > mx ? d32 : mx;
+        sum += d;
+        uint64_t sq = d * d;
+        lo += sq & 0xffffffffull;
+        hi += sq >> 32;
+    }
+    mx = wave_max_u32(mx);
+    sum = wave_sum_u64(sum);
+    lo = wave_sum_u64(lo);
+    hi = wave_sum_u64(hi);
+    if (lane_id() == 0) {
+        atomicMax(&out[0], (unsigned long long)mx);
+        atomicAdd(&out[1], (unsigned long long)sum);
+        atomicAdd(&out[2], (unsigned long long)lo);
+        atomicAdd(&out[3], (unsigned long long)hi);
+    }
+}
+
 } // namespace
 
 size_t compact_tmp_bytes(uint32_t V, uint32_t Vn)
@@ -351,6 +425,47 @@ hipError_t scatter_cores(const uint32_t *sub, const uint32_t *gmap, uint32_t n, 
         return hipSuccess;
     hipLaunchKernelGGL(scatter_kernel, dim3((n + CB - 1) / CB), dim3(CB), 0, st, sub, gmap, n,
                        core);
+    return hipGetLastError();
+}
+
+hipError_t kcore_exclusion(const uint32_t *core, uint32_t V, uint32_t k, uint64_t *excl,
+                           hipStream_t st)
+{
+    uint32_t nw = (V + 63) / 64;
+    if (nw == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kcore_mark, dim3((nw * 64 + CB - 1) / CB), dim3(CB), 0, st, core, V, k,
+                       excl, nw);
+    return hipGetLastError();
+}
+
+hipError_t kcore_row_lengths(const void *rowptr, bool wide, const uint32_t *col,
+                             const uint64_t *excl, uint32_t V, uint32_t *deg, hipStream_t st)
+{
+    if (V == 0)
+        return hipSuccess;
+    uint32_t g = std::min<uint32_t>(8192, (V + (CB / KC_WAVE) - 1) / (CB / KC_WAVE));
+    if (wide)
+        hipLaunchKernelGGL(count_live<uint64_t>, dim3(g), dim3(CB), 0, st,
+                           (const uint64_t *)rowptr, col, excl, V, deg);
+    else
+        hipLaunchKernelGGL(count_live<uint32_t>, dim3(g), dim3(CB), 0, st,
+                           (const uint32_t *)rowptr, col, excl, V, deg);
+    return hipGetLastError();
+}
+
+hipError_t degree_stats(const void *rowptr, bool wide, uint32_t V, unsigned long long *out,
+                        hipStream_t st)
+{
+    if (V == 0)
+        return hipSuccess;
+    uint32_t g = std::min<uint32_t>(1024, (V + CB - 1) / CB);
+    if (wide)
+        hipLaunchKernelGGL(degree_stats_kernel<uint64_t>, dim3(g), dim3(CB), 0, st,
+                           (const uint64_t *)rowptr, V, out);
+    else
+        hipLaunchKernelGGL(degree_stats_kernel<uint32_t>, dim3(g), dim3(CB), 0, st,
+                           (const uint32_t *)rowptr, V, out);
     return hipGetLastError();
 }
 

@@ -40,6 +40,16 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x)
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t y = __shfl_xor(x, o, KC_WAVE);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x)
 {
 #pragma unroll

@@ -103,6 +103,16 @@ size_t compact_tmp_bytes(uint32_t V, uint32_t Vn);
 // Phase 2: the adjacency.  A failed consistency check sets bad[0]; the next
 // peel stage takes bad as its abort flag.
 hipError_t compact_cols(const CompactIn &in, const CompactOut &out, hipStream_t st);
+// k-core subgraph: bit set = vertex excluded (core < k or past V), one bit per
+// vertex in (V+63)/64 words; row lengths = live neighbours of live vertices.
+hipError_t kcore_exclusion(const uint32_t *core, uint32_t V, uint32_t k, uint64_t *excl,
+                           hipStream_t st);
+hipError_t kcore_row_lengths(const void *rowptr, bool wide, const uint32_t *col,
+                             const uint64_t *excl, uint32_t V, uint32_t *deg, hipStream_t st);
+// out[4] (zeroed by the caller): max degree, degree sum, and the sums of the
+// low and high 32-bit halves of the squared degrees
+hipError_t degree_stats(const void *rowptr, bool wide, uint32_t V, unsigned long long *out,
+                        hipStream_t st);
 // core[gmap[i]] = sub[i] for i < n
 hipError_t scatter_cores(const uint32_t *sub, const uint32_t *gmap, uint32_t n, uint32_t *core,
                          hipStream_t st);

@@ -241,6 +241,8 @@ def lib():
         "kc_update": ([_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp,
                        ctypes.POINTER(UpdateStats)], S),
         "kc_graph_dims": ([_vp, _u32p, _u64p], S),
+        "kc_graph_degree_stats": ([_vp, _vp], S),
+        "kc_kcore_subgraph": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp), _vp, _u32p], S),
         "kc_server_create": ([ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(_vp)], S),
         "kc_server_gen_rmat_graph": ([_vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
                                       _u32p], S),
@@ -529,7 +531,29 @@ class Context:
         return DeviceGraph(self, h, len(rp) - 1)
 
 
+class DegreeStats(ctypes.Structure):
+    _fields_ = [("V", ctypes.c_uint32), ("dmax", ctypes.c_uint32), ("nnz", ctypes.c_uint64),
+                ("mean", ctypes.c_double), ("stddev", ctypes.c_double),
+                ("reserved", ctypes.c_uint64 * 2)]
+
+
 class DeviceGraph:
+    def degree_stats(self):
+        """V, max degree, nnz, mean and population standard deviation of the
+        degrees (what peelone_dyn prints: main.cu:10-63)."""
+        ds = DegreeStats()
+        _check(lib().kc_graph_degree_stats(self._h, ctypes.byref(ds)))
+        return {k: getattr(ds, k) for k, _ in ds._fields_ if k != "reserved"}
+
+    def kcore_subgraph(self, k):
+        """The k-core (core >= k, from the last decomposition) as a new device
+        graph, and the original id of each of its vertices."""
+        ids = np.zeros(max(self.V, 1), np.uint32)
+        n = ctypes.c_uint32(0)
+        h = _vp()
+        _check(lib().kc_kcore_subgraph(self._h, k, ctypes.byref(h), _ptr(ids), ctypes.byref(n)))
+        return DeviceGraph(self.ctx, h, n.value), ids[:n.value]
+
     def autotune(self, method="peelone_dyn", reps=1):
         """Time the launch candidates of `method` on this graph and keep the
         fastest as the graph's default; returns (opts dict, ms)."""
