@@ -125,6 +125,52 @@ __global__ void dyn_edit_kernel(DynView d, const uint32_t *eu, const uint32_t *e
     }
 }
 
+// Batched deletion: one wave per edited row x, which removes every neighbour
+// listed for it (hn[hs[j] .. hs[j+1]), sorted) in one pass over the row,
+// compacting in place (kept entries only move towards the front, after the
+// wave has read them).  Each row is edited by exactly one wave, so no
+// matching split is needed.  A neighbour found sets its edge's status (both
+// half-edges of an edge are found or neither: the rows are symmetric).
+__global__ void dyn_delete_rows_kernel(DynView d, const uint32_t *hv, const uint32_t *hs,
+                                       uint32_t nseg, const uint32_t *hn, const uint32_t *he,
+                                       uint8_t *status)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t j = wave_id_global(); j < nseg; j += waves_total()) {
+        const uint32_t x = hv[j], b = hs[j], cnt = hs[j + 1] - b;
+        const uint32_t *del = hn + b;
+        uint32_t *row = d.adj + d.off[x];
+        const uint32_t n = d.len[x];
+        uint32_t out = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += KC_WAVE) {
+            uint32_t i = i0 + lane;
+            uint32_t y = i < n ? row[i] : 0u;
+            bool gone = false;
+            if (i < n) { // binary search in the sorted deletions
+                uint32_t lo = 0, hi = cnt;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (del[mid] < y)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < cnt && del[lo] == y) {
+                    gone = true;
+                    status[he[b + lo]] = 1;
+                }
+            }
+            bool keep = i < n && !gone;
+            uint64_t m = ballot(keep);
+            if (keep)
+                row[out + lane_rank(m)] = y; // out + rank <= i: already read
+            out += (uint32_t)__popcll(m);
+        }
+        if (lane == 0)
+            d.len[x] = out;
+    }
+}
+
 // roots of the applied edges: endpoint(s) holding the edge's smaller core
 __global__ void dyn_roots_kernel(DynView d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
                                  const uint8_t *status, uint32_t *mark, uint32_t stamp,
@@ -281,6 +327,16 @@ hipError_t dyn_edit(const DynView &d, const uint32_t *eu, const uint32_t *ev, ui
     if (n)
         hipLaunchKernelGGL(dyn_edit_kernel, wave_grid(n), dim3(DYN_BLOCK), 0, st, d, eu, ev, n,
                            op, status);
+    return hipGetLastError();
+}
+
+hipError_t dyn_delete_rows(const DynView &d, const uint32_t *hv, const uint32_t *hs,
+                          uint32_t nseg, const uint32_t *hn, const uint32_t *he, uint8_t *status,
+                          hipStream_t st)
+{
+    if (nseg)
+        hipLaunchKernelGGL(dyn_delete_rows_kernel, wave_grid(nseg), dim3(DYN_BLOCK), 0, st, d, hv,
+                           hs, nseg, hn, he, status);
     return hipGetLastError();
 }
 

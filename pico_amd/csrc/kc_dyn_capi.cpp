@@ -3,11 +3,12 @@
 // reference, which only decomposes static graphs).
 //
 // Per call: the batch is validated and normalised on the host (ids < V,
-// self-loops dropped, each pair once), split into matchings (no vertex twice
-// in one matching), and applied matching by matching on the slack CSR.
-// Insertions maintain the cores after every matching (raise the candidate
-// region by one, lower to the fixed point); deletions apply every matching,
-// then lower from the roots of all of them.
+// self-loops dropped, each pair once).  Insertions are split into matchings
+// (no vertex twice in one matching) and applied matching by matching on the
+// slack CSR, the cores maintained after every matching (raise the candidate
+// region by one, lower to the fixed point).  Deletions are applied in one
+// launch, one wave per edited row, then lowered from the roots of all of
+// them.
 #include "kc_capi_impl.hpp"
 
 #include <algorithm>
@@ -28,6 +29,10 @@ struct DynState {
     uint32_t *eu = nullptr, *ev = nullptr;
     uint8_t *status = nullptr;
     uint32_t ecap = 0;
+    // batched deletions: per edited row its vertex and segment start, and
+    // per half-edge the neighbour to drop and the edge's index
+    uint32_t *hv = nullptr, *hs = nullptr, *hn = nullptr, *he = nullptr;
+    uint64_t hcap = 0;
     uint64_t slots = 0;
     uint32_t stamp = 0;
     std::vector<uint64_t> off_h;
@@ -41,7 +46,8 @@ void dyn_free(DynState *d)
     for (void *p : {(void *)d->off, (void *)d->len, (void *)d->adj, (void *)d->core,
                     (void *)d->core_prev, (void *)d->mark, (void *)d->visit, (void *)d->list[0],
                     (void *)d->list[1], (void *)d->list[2], (void *)d->cnt, (void *)d->eu,
-                    (void *)d->ev, (void *)d->status})
+                    (void *)d->ev, (void *)d->status, (void *)d->hv, (void *)d->hs,
+                    (void *)d->hn, (void *)d->he})
         if (p)
             hipFree(p);
     delete d;
@@ -375,10 +381,21 @@ extern "C" kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u,
     }
     DynState *d = g->dyn;
     // greedy split into matchings: edge -> first matching after both ends'
+    // (insertions only; deletions edit every row in one launch)
     std::unordered_map<uint32_t, uint32_t> next;
-    next.reserve(keys.size() * 2);
+    next.reserve(op == KC_INSERT ? keys.size() * 2 : 0);
     std::vector<std::vector<uint32_t>> mu, mv;
-    for (uint64_t k : keys) {
+    if (op == KC_DELETE && !keys.empty()) {
+        mu.emplace_back();
+        mv.emplace_back();
+        mu[0].reserve(keys.size());
+        mv[0].reserve(keys.size());
+        for (uint64_t k : keys) {
+            mu[0].push_back((uint32_t)(k >> 32));
+            mv[0].push_back((uint32_t)k);
+        }
+    }
+    for (uint64_t k : op == KC_INSERT ? keys : std::vector<uint64_t>{}) {
         uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
         auto ia = next.find(a), ib = next.find(b);
         uint32_t m = std::max(ia == next.end() ? 0u : ia->second,
@@ -458,26 +475,72 @@ extern "C" kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u,
             if ((s = lower(d, nc, us, st)))
                 return s;
         }
-    } else {
-        // every matching's edits first; the roots of all of them seed one
-        // lowering pass (the old cores are an upper bound of the new ones)
+    } else if (!mu.empty()) {
+        // every deletion in one launch (rows grouped by the vertex that owns
+        // them), then one lowering pass from the roots of all of them (the
+        // old cores are an upper bound of the new ones)
+        const uint32_t k = (uint32_t)mu[0].size();
+        std::vector<std::pair<uint64_t, uint32_t>> half;
+        half.reserve(2ull * k);
+        for (uint32_t i = 0; i < k; i++) {
+            half.push_back({(uint64_t)mu[0][i] << 32 | mv[0][i], i});
+            half.push_back({(uint64_t)mv[0][i] << 32 | mu[0][i], i});
+        }
+        std::sort(half.begin(), half.end());
+        std::vector<uint32_t> hv, hs, hn(half.size()), he(half.size());
+        for (size_t j = 0; j < half.size(); j++) {
+            uint32_t x = (uint32_t)(half[j].first >> 32);
+            if (hv.empty() || hv.back() != x) {
+                hv.push_back(x);
+                hs.push_back((uint32_t)j);
+            }
+            hn[j] = (uint32_t)half[j].first;
+            he[j] = half[j].second;
+        }
+        hs.push_back((uint32_t)half.size());
+        if (half.size() + 1 > d->hcap) {
+            for (uint32_t **p : {&d->hv, &d->hs, &d->hn, &d->he}) {
+                hipFree(*p);
+                *p = nullptr;
+            }
+            d->hcap = 0;
+            uint64_t c = half.size() + 1;
+            if ((s = dmalloc(d, (void **)&d->hv, c * 4, "deletion rows")) ||
+                (s = dmalloc(d, (void **)&d->hs, c * 4, "deletion rows")) ||
+                (s = dmalloc(d, (void **)&d->hn, c * 4, "deletion rows")) ||
+                (s = dmalloc(d, (void **)&d->he, c * 4, "deletion rows")))
+                return s;
+            d->hcap = c;
+        }
+        const uint32_t nseg = (uint32_t)hv.size();
+        KC_HIP(hipMemcpyAsync(d->eu, mu[0].data(), k * 4ull, hipMemcpyHostToDevice, st), "edges");
+        KC_HIP(hipMemcpyAsync(d->ev, mv[0].data(), k * 4ull, hipMemcpyHostToDevice, st), "edges");
+        KC_HIP(hipMemcpyAsync(d->hv, hv.data(), nseg * 4ull, hipMemcpyHostToDevice, st), "rows");
+        KC_HIP(hipMemcpyAsync(d->hs, hs.data(), (nseg + 1ull) * 4, hipMemcpyHostToDevice, st),
+               "rows");
+        KC_HIP(hipMemcpyAsync(d->hn, hn.data(), hn.size() * 4, hipMemcpyHostToDevice, st),
+               "rows");
+        KC_HIP(hipMemcpyAsync(d->he, he.data(), he.size() * 4, hipMemcpyHostToDevice, st),
+               "rows");
+        KC_HIP(hipMemsetAsync(d->status, 0, k, st), "status clear");
+        KC_HIP(kc::dyn_delete_rows(view(d), d->hv, d->hs, nseg, d->hn, d->he, d->status, st),
+               "edge deletion");
         uint32_t rs = next_stamp(d, st);
         KC_HIP(hipMemsetAsync(d->cnt, 0, 4, st), "counter clear");
-        for (size_t m = 0; m < mu.size(); m++) {
-            uint32_t k = (uint32_t)mu[m].size();
-            KC_HIP(hipMemcpyAsync(d->eu, mu[m].data(), k * 4ull, hipMemcpyHostToDevice, st),
-                   "edges");
-            KC_HIP(hipMemcpyAsync(d->ev, mv[m].data(), k * 4ull, hipMemcpyHostToDevice, st),
-                   "edges");
-            if ((s = edit_matching(g, d, 1, k, mu[m], mv[m], stat, us, st)))
-                return s;
-            KC_HIP(kc::dyn_roots(view(d), d->eu, d->ev, k, d->status, d->mark, rs, d->list[0],
-                                 d->cnt, V, st),
-                   "roots");
-        }
+        KC_HIP(kc::dyn_roots(view(d), d->eu, d->ev, k, d->status, d->mark, rs, d->list[0],
+                             d->cnt, V, st),
+               "roots");
+        stat.resize(k);
+        KC_HIP(hipMemcpyAsync(stat.data(), d->status, k, hipMemcpyDeviceToHost, st),
+               "status copy");
         uint32_t nr = 0;
         if ((s = read_u32(d->cnt, nr, st)))
             return s;
+        uint64_t applied = 0;
+        for (uint8_t x : stat)
+            applied += x == 1;
+        us.applied += applied;
+        g->nnz -= 2 * applied;
         if ((s = lower(d, nr, us, st)))
             return s;
     }

@@ -200,6 +200,12 @@ struct DynView {
 // op 0 insert / 1 delete of a matching; status[i]: 0 skipped, 1 applied, 2 row full
 hipError_t dyn_edit(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
                     uint32_t op, uint8_t *status, hipStream_t st);
+// every deletion at once: segment j lists (sorted) the neighbours hn[hs[j] ..
+// hs[j+1]) to drop from row hv[j] and their edges' indexes he[]; status[e]
+// becomes 1 for each edge found (status zeroed by the caller)
+hipError_t dyn_delete_rows(const DynView &d, const uint32_t *hv, const uint32_t *hs,
+                          uint32_t nseg, const uint32_t *hn, const uint32_t *he, uint8_t *status,
+                          hipStream_t st);
 hipError_t dyn_roots(const DynView &d, const uint32_t *eu, const uint32_t *ev, uint32_t n,
                      const uint8_t *status, uint32_t *mark, uint32_t stamp, uint32_t *list,
                      uint32_t *n_list, uint32_t cap, hipStream_t st);

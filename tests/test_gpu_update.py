@@ -127,6 +127,17 @@ def test_update_hub_growth(ctx):
     core, st = dg.update("delete", a, b)
     rp, col = es.csr()
     np.testing.assert_array_equal(core, oracle.bz(rp, col))
+    # deletions go in one launch, however many share a vertex: the hub's 600
+    # new edges plus some it had before, in scrambled order
+    old = col[rp[0]:rp[1]]
+    dv = np.random.default_rng(8).permutation(old).astype(np.uint32)
+    du = np.zeros_like(dv)
+    es.apply("delete", dv, du)
+    core, st = dg.update("delete", dv, du)
+    assert st["matchings"] == 1 and st["applied"] == len(dv)
+    rp, col = es.csr()
+    assert rp[1] == rp[0]
+    np.testing.assert_array_equal(core, oracle.bz(rp, col))
     dg.close()
 
 
