@@ -391,6 +391,10 @@ typedef struct kc_update_stats {
 
 KC_API kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u, const uint32_t *v,
                            uint64_t n, uint32_t *core_out, kc_update_stats *stats);
+/* Build the maintenance state (cores by a bucket peel, slack adjacency) now
+ * instead of in the first kc_update: a server calls it when a graph becomes
+ * resident, so that no request pays for it.  No reference counterpart. */
+KC_API kc_status kc_update_prepare(kc_graph *g);
 /* current vertex count and adjacency entries (2x undirected edges) */
 KC_API kc_status kc_graph_dims(const kc_graph *g, uint32_t *V, uint64_t *nnz);
 KC_API kc_status kc_graph_degree_stats(kc_graph *g, kc_degree_stats *out);

@@ -565,6 +565,20 @@ extern "C" kc_status kc_update(kc_graph *g, kc_update_op op, const uint32_t *u,
     return KC_OK;
 }
 
+extern "C" kc_status kc_update_prepare(kc_graph *g)
+{
+    if (!g)
+        return fail(KC_EINVAL, "null graph");
+    if (!g->owns_csr)
+        return fail(KC_EINVAL, "graph is a view into a batch");
+    if (g->dyn)
+        return KC_OK;
+    if (g->csr_stale)
+        return fail(KC_EDEVICE, "stale CSR without dynamic state");
+    KC_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    return dyn_create(g);
+}
+
 extern "C" kc_status kc_graph_dims(const kc_graph *g, uint32_t *V, uint64_t *nnz)
 {
     if (!g)

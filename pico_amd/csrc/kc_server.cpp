@@ -351,7 +351,13 @@ kc_status kc_server_gen_rmat_graph(kc_server *s, uint32_t scale, double edge_fac
     kc_graph *g = nullptr;
     uint64_t m_gen = (uint64_t)std::llround(edge_factor * (double)(1ull << scale));
     st = kc_graph_gen_rmat(c, scale, m_gen, RMAT_A, RMAT_B, RMAT_C, seed, 4, &g);
+    // resident graphs take updates: build their maintenance state now, not in
+    // the first update request
+    if (!st)
+        st = kc_update_prepare(g);
     if (st) {
+        if (g)
+            kc_graph_free(g);
         kc_ctx_destroy(c);
         return st;
     }

@@ -242,6 +242,7 @@ def lib():
                        ctypes.POINTER(UpdateStats)], S),
         "kc_graph_dims": ([_vp, _u32p, _u64p], S),
         "kc_graph_degree_stats": ([_vp, _vp], S),
+        "kc_update_prepare": ([_vp], S),
         "kc_kcore_subgraph": ([_vp, ctypes.c_uint32, ctypes.POINTER(_vp), _vp, _u32p], S),
         "kc_server_create": ([ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(_vp)], S),
         "kc_server_gen_rmat_graph": ([_vp, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
@@ -580,6 +581,10 @@ class DeviceGraph:
         n = ctypes.c_uint64(0)
         _check(lib().kc_graph_dims(self._h, None, ctypes.byref(n)))
         return n.value
+
+    def update_prepare(self):
+        """Build the maintenance state now rather than in the first update."""
+        _check(lib().kc_update_prepare(self._h))
 
     def update(self, op, u, v, copy=True):
         """Insert (op "insert") or delete ("delete") the unordered pairs

@@ -185,3 +185,19 @@ def test_update_isolated_and_empty(ctx):
         rp, col = es.csr()
         np.testing.assert_array_equal(core, oracle.bz(rp, col))
     dg.close()
+
+
+def test_update_prepare_then_update(ctx):
+    hg = pa.HostGraph.rmat(11, 8, seed=12)
+    es = EdgeSet(hg)
+    dg = ctx.upload(hg)
+    dg.update_prepare()
+    dg.update_prepare()  # idempotent
+    rng = np.random.default_rng(13)
+    u = rng.integers(0, hg.V, 300).astype(np.uint32)
+    v = rng.integers(0, hg.V, 300).astype(np.uint32)
+    es.apply("insert", u, v)
+    core, st = dg.update("insert", u, v)
+    rp, col = es.csr()
+    np.testing.assert_array_equal(core, oracle.bz(rp, col))
+    dg.close()
