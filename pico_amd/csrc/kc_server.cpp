@@ -81,7 +81,9 @@ uint64_t rmat_m_gen(const kc_request &r)
 
 // HBM footprint estimate: the device build (two sort buffers of 2 keys per
 // generated edge, the unsorted column bound, row offsets) plus the engine's
-// per-vertex scratch
+// per-vertex scratch.  The build's buffers are released before the peel
+// runs, so its peak also covers the peel's compacted stages (two sets of at
+// most 8 bytes per undirected edge, 16 per vertex).
 uint64_t footprint(const kc_server *s, const kc_request &r, size_t nedges)
 {
     switch (r.kind) {
@@ -92,8 +94,9 @@ uint64_t footprint(const kc_server *s, const kc_request &r, size_t nedges)
     case KC_REQ_UPDATE:
         return 16 * (uint64_t)nedges + (1ull << 20);
     case KC_REQ_DECOMPOSE_GRAPH: {
+        // per-vertex scratch, plus the compacted stages of a bucket peel
         const kc_graph *g = s->graphs[r.graph].g;
-        return 40ull * g->V + (1ull << 20);
+        return 72ull * g->V + 8ull * g->nnz + (1ull << 20);
     }
     default:
         return 0;
