@@ -787,8 +787,11 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         (s = ensure(g, g->kmax, 8, "kmax")))
         return s;
     if (method != KC_HISTO &&
-        ((s = ensure(g, g->out, V4, "new values")) || (s = ensure(g, g->flags, V4, "marks")) ||
-         (s = ensure(g, g->hl2, V4, "list"))))
+        ((s = ensure(g, g->out, V4, "new values")) || (s = ensure(g, g->flags, V4, "marks"))))
+        return s;
+    // CntCore / NbrCore: a third list; HistoCore: its list of vertices of more
+    // than KC_TINY neighbours (the fourth list of the kernel's arguments)
+    if ((s = ensure(g, g->hl2, V4, "list")))
         return s;
     if (method == KC_CNT &&
         ((s = ensure(g, g->cnt, V4, "cnt")) || (s = ensure(g, g->list0, V4, "list"))))
@@ -837,7 +840,11 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.list0 = (uint32_t *)g->hl0.p;
     p.list1 = (uint32_t *)g->hl1.p;
     p.list2 = (uint32_t *)g->hl2.p;
-    p.list3 = (uint32_t *)g->list0.p;
+    p.list3 = method == KC_HISTO ? (uint32_t *)g->hl2.p : (uint32_t *)g->list0.p;
+    // every list the method writes must exist before the launch
+    if (!p.list0 || !p.list1 || (method != KC_HISTO && !p.list2) ||
+        (method != KC_NBR && !p.list3) || (method == KC_CNT && !p.cnt))
+        return fail(KC_EDEVICE, "h-index work lists not allocated");
     p.chunks = (uint64_t *)g->list1.p;
     p.chunk_cap = (uint32_t)chunk_cap;
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;

@@ -33,6 +33,9 @@ enum { HM_HISTO = 0, HM_CNT = 1, HM_NBR = 2 };
 
 constexpr uint32_t KC_LBINS = 512;  // LDS histogram bins per wavefront
 constexpr uint32_t KC_HBIG = 4096;  // items above this go to the workgroup
+// HistoCore: vertices of at most this many neighbours get a thread each (not a
+// wave) for their initial histogram and their first count
+constexpr uint32_t KC_TINY = 16;
 constexpr uint32_t KC_HGRAB = 16;   // items taken per wavefront grab
 constexpr uint32_t KC_BIGQ = 256;   // per-WG queue of big items
 constexpr uint32_t KC_HCH = 64;     // edges per virtual node of a changed vertex
@@ -470,10 +473,53 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 
     // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
     //      (histo_kernel, HistoCore.cu:15-75)
+    uint32_t n_big = V; // HistoCore: vertices of more than KC_TINY neighbours
     if (a.method == HM_HISTO) {
+        // Most vertices of a power-law graph have a handful of neighbours: a
+        // thread builds such a histogram in registers and stores its d+1
+        // words; the others are listed for the wave / workgroup builders.
         uint32_t phase = gen;
+        {
+            const uint32_t nthr = gridDim.x * BLOCK;
+            const uint32_t t0 = blockIdx.x * BLOCK + (threadIdx.x & ~(KC_WAVE - 1));
+            for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+                uint32_t v = v0 + lane;
+                bool valid = v < V;
+                uint32_t d = valid ? (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]) : 0u;
+                bool big = valid && d > KC_TINY;
+                if (valid && !big) {
+                    uint64_t s0 = a.rowptr[v], b = s0 + v;
+                    uint32_t x[KC_TINY];
+#pragma unroll
+                    for (uint32_t k = 0; k < KC_TINY; k++)
+                        x[k] = k < d ? a.col[s0 + k] : 0u;
+#pragma unroll
+                    for (uint32_t k = 0; k < KC_TINY; k++)
+                        if (k < d) {
+                            uint32_t y = a.core[x[k]];
+                            x[k] = y < d ? y : d;
+                        }
+                    a.histo[b] = d; // 0 for an isolated vertex, as the reference's zeroing
+#pragma unroll
+                    for (uint32_t j = 1; j <= KC_TINY; j++) {
+                        if (j > d)
+                            break;
+                        uint32_t c = 0;
+#pragma unroll
+                        for (uint32_t k = 0; k < KC_TINY; k++)
+                            c += (k < d && x[k] == j) ? 1u : 0u;
+                        a.histo[b + j] = c;
+                    }
+                }
+                hpush(gs, phase, a.list[3], cap_list, big, v);
+            }
+        }
+        if (!hbar(a, gs, gen, round, 0))
+            goto out;
+        n_big = ld_relaxed(&slot_of(gs, phase)[0].qtail);
+        phase = gen;
         for_items<BLOCK, R>(
-            L, a, phase, V, nullptr,
+            L, a, phase, n_big, a.list[3],
             [&](uint32_t v, uint32_t d) {
                 uint64_t b = hbase(a.rowptr, v);
                 if (d == 0) { // the reference zeroes the whole array first (:286)
@@ -516,6 +562,10 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         uint32_t *c_list = a.list[1];
         if (round == 1) {
             n_in = V; // every vertex; the per-method filter is applied below
+            if (a.method == HM_HISTO) { // tiny vertices first, thread by thread
+                n_in = n_big;
+                in_list = a.list[3];
+            }
         } else {
             n_in = L.done; // length of the list the previous phase built
             in_list = a.method == HM_CNT ? act_cur : a.list[0];
@@ -576,6 +626,75 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 n_st = 0;
                 st_nch = 0;
             };
+            if (round == 1) {
+                // the first count of the vertices of at most KC_TINY
+                // neighbours, a thread each: bins from the old core (its
+                // degree) down; a change collapses the bins as above and
+                // queues the vertex's one virtual node (at most KC_TINY < KC_HCH
+                // edges), unless no neighbour can hold a value above k
+                const uint32_t nthr = gridDim.x * BLOCK;
+                const uint32_t t0 = blockIdx.x * BLOCK + (threadIdx.x & ~(KC_WAVE - 1));
+                PhaseSlot *ps = &slot_of(gs, phase)[0];
+                for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+                    uint32_t v = v0 + lane;
+                    bool chg = false, q = false;
+                    if (v < V) {
+                        uint64_t s0 = a.rowptr[v];
+                        uint32_t d = (uint32_t)(a.rowptr[v + 1] - s0);
+                        uint32_t old = a.core[v];
+                        if (d <= KC_TINY && old > 1) {
+                            uint64_t b = s0 + v;
+                            uint32_t bin[KC_TINY + 1];
+#pragma unroll
+                            for (uint32_t j = 1; j <= KC_TINY; j++)
+                                bin[j] = j <= old ? a.histo[b + j] : 0u;
+                            uint32_t sum = 0, k = 0;
+                            bool hit = false;
+#pragma unroll
+                            for (uint32_t j = KC_TINY; j >= 1; j--) {
+                                if (!hit && j <= old) {
+                                    sum += bin[j];
+                                    if (sum >= j) {
+                                        hit = true;
+                                        k = j;
+                                    }
+                                }
+                            }
+                            if (k != old) {
+                                a.histo[b] = old;
+                                a.histo[b + k] = sum;
+                                a.core[v] = k;
+                                chg = true;
+                                chg_deg += d;
+                                uint32_t r = d;
+                                if (a.nd_ds) {
+                                    r = 0;
+                                    for (uint32_t j = 0; j < d; j++)
+                                        r += a.nd_ds[s0 + j] > k ? 1u : 0u;
+                                }
+                                q = r > 0;
+                            }
+                        }
+                    }
+                    uint64_t cm = ballot(chg), qm = ballot(q);
+                    uint32_t base = 0;
+                    if (lane == 0) {
+                        if (cm)
+                            atomicAdd((unsigned long long *)&ps->sum0,
+                                      (unsigned long long)__popcll(cm));
+                        if (qm)
+                            base = atomicAdd(&ps->qtail, (uint32_t)__popcll(qm));
+                    }
+                    base = __shfl(base, 0, KC_WAVE);
+                    if (q) {
+                        uint32_t p = base + lane_rank(qm);
+                        if (p < a.chunk_cap)
+                            a.chunks[p] = (uint64_t)v; // virtual node 0 of v
+                        else
+                            sync_fail(gs, 2);
+                    }
+                }
+            }
             for_items_g<BLOCK, R>(L, a, phase, n_in, in_list, one,
                                 [&](uint32_t v) {
                                     uint32_t old = a.core[v];
