@@ -315,7 +315,8 @@ def summarize_pmc(paths, peel_stages=1):
     sw = COPY_PROBE_BYTES / (cw * 1024.0)
     out = {"copy_probe": {"bytes": COPY_PROBE_BYTES, "fetch_kib": cf, "write_kib": cw,
                           "fetch_scale": sf, "write_scale": sw},
-           "sources": [os.path.relpath(p, ROOT) for p in paths]}
+           "sources": ["FETCH_SIZE pass: " + os.path.basename(paths[0]),
+                       "WRITE_SIZE pass: " + os.path.basename(paths[1])]}
     for key, kname in (("peel", "peel_kernel"), ("hindex", "hindex_kernel"),
                        ("batch", "batch_kernel")):
         f, w = pick(fetch, kname), pick(write, kname)
@@ -334,9 +335,9 @@ def summarize_pmc(paths, peel_stages=1):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one process on one MI355X "
-              "(see DESIGN.md, 'How the path would shard'); --gpus %d is not supported"
-              % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs one process on one MI355X and "
+              "has no multi-GPU sharding; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
