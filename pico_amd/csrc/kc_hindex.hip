@@ -908,8 +908,9 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             for_items<BLOCK, R>(
                 L, a, phase, n_changed, c_list,
                 [&](uint32_t v, uint32_t d) {
+                    const uint32_t nc = a.newc[v];
                     if (lane == 0)
-                        a.core[v] = a.newc[v];
+                        a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
                     for (uint64_t i0 = s; i0 < e; i0 += 4 * KC_WAVE) {
                         uint32_t u[4];
@@ -924,7 +925,14 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             fresh[k] = false;
-                            if (u[k] != 0xffffffffu && a.mark[u[k]] != epoch) {
+                            // NbrCore: u's next h-index clamps its neighbours
+                            // at its own core, so v's drop to nc matters to u
+                            // only when nc < core[u] (read old or new: either
+                            // way no affected u is missed), and an unaffected
+                            // u would recompute its present value
+                            if (u[k] != 0xffffffffu &&
+                                (a.method != HM_NBR || a.core[u[k]] > nc) &&
+                                a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
                                 if (fresh[k] && a.method == HM_CNT)
@@ -937,8 +945,10 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     }
                 },
                 [&](uint32_t v) {
+                    const uint32_t nc = a.newc[v];
+                    __syncthreads(); // every thread has read newc[v] before it is applied
                     if (threadIdx.x == 0)
-                        a.core[v] = a.newc[v];
+                        a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
                     for (uint64_t i0 = s; i0 < e; i0 += 4 * (uint64_t)BLOCK) {
                         uint32_t u[4];
@@ -951,7 +961,14 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             fresh[k] = false;
-                            if (u[k] != 0xffffffffu && a.mark[u[k]] != epoch) {
+                            // NbrCore: u's next h-index clamps its neighbours
+                            // at its own core, so v's drop to nc matters to u
+                            // only when nc < core[u] (read old or new: either
+                            // way no affected u is missed), and an unaffected
+                            // u would recompute its present value
+                            if (u[k] != 0xffffffffu &&
+                                (a.method != HM_NBR || a.core[u[k]] > nc) &&
+                                a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
                                 if (fresh[k] && a.method == HM_CNT)
