@@ -905,10 +905,24 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         {
             uint32_t *m_list = a.method == HM_NBR ? a.list[0] : a.list[2];
             uint32_t epoch = round;
+            // Which neighbours u of v (core ov -> nc) to mark.  core[u] may be
+            // read before or after u's own update of this phase; both choices
+            // are safe.  NbrCore: u's next h-index clamps neighbours at u's
+            // core, so only nc < core[u] can lower it; an unmarked u would
+            // recompute its present value.  CntCore: cnt[u] counts neighbours
+            // at or above core[u], which v leaves only when
+            // nc < core[u] <= ov.  An unmarked u keeps a cnt that is at most
+            // its true count (a neighbour it counted and lost satisfies that
+            // test for either read of core[u]), so every vertex with
+            // cnt < core stays active and each round changes the same set.
+            auto affects = [&](uint32_t u, uint32_t nc, uint32_t ov) -> bool {
+                uint32_t cu = a.core[u];
+                return nc < cu && (a.method == HM_NBR || cu <= ov);
+            };
             for_items<BLOCK, R>(
                 L, a, phase, n_changed, c_list,
                 [&](uint32_t v, uint32_t d) {
-                    const uint32_t nc = a.newc[v];
+                    const uint32_t nc = a.newc[v], ov = a.core[v];
                     if (lane == 0)
                         a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
@@ -925,13 +939,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             fresh[k] = false;
-                            // NbrCore: u's next h-index clamps its neighbours
-                            // at its own core, so v's drop to nc matters to u
-                            // only when nc < core[u] (read old or new: either
-                            // way no affected u is missed), and an unaffected
-                            // u would recompute its present value
-                            if (u[k] != 0xffffffffu &&
-                                (a.method != HM_NBR || a.core[u[k]] > nc) &&
+                            if (u[k] != 0xffffffffu && affects(u[k], nc, ov) &&
                                 a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
@@ -945,8 +953,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     }
                 },
                 [&](uint32_t v) {
-                    const uint32_t nc = a.newc[v];
-                    __syncthreads(); // every thread has read newc[v] before it is applied
+                    const uint32_t nc = a.newc[v], ov = a.core[v];
+                    __syncthreads(); // every thread has read core[v] before it is replaced
                     if (threadIdx.x == 0)
                         a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
@@ -961,13 +969,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             fresh[k] = false;
-                            // NbrCore: u's next h-index clamps its neighbours
-                            // at its own core, so v's drop to nc matters to u
-                            // only when nc < core[u] (read old or new: either
-                            // way no affected u is missed), and an unaffected
-                            // u would recompute its present value
-                            if (u[k] != 0xffffffffu &&
-                                (a.method != HM_NBR || a.core[u[k]] > nc) &&
+                            if (u[k] != 0xffffffffu && affects(u[k], nc, ov) &&
                                 a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
