@@ -217,15 +217,16 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
     __threadfence_block();
 }
 
-// h-index of v's neighbour values capped at cap, computed by one wave.
+// h-index k of v's neighbour values capped at cap, computed by one wave, and
+// the number of neighbours whose value is at least k.
 template <typename R, int BLOCK>
 __device__ __forceinline__ uint32_t wave_hindex(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v,
-                                                uint32_t cap)
+                                                uint32_t cap, uint32_t &sum)
 {
+    sum = 0;
     if (cap == 0)
         return 0;
     uint32_t w = threadIdx.x / KC_WAVE;
-    uint32_t sum;
     if (cap < KC_LBINS) {
         wave_hist<R, true>(a, v, cap, a.core, L.bins[w]);
         return wave_hsearch(L.bins[w], cap, sum);
@@ -739,7 +740,15 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             // NbrCore count_kernel (NbrCore.cu:78-113) / CntCore count_kernel
             // (CntCore.cu:82-130): recompute from scratch; the new value is
             // deferred to newc[] so every vertex of this round reads the
-            // previous round's values
+            // previous round's values.
+            // cnt_refresh: a CntCore vertex whose value holds gets cnt = its
+            // neighbours at or above its core (>= core), so it leaves the
+            // active set until a neighbour crosses its core (the reference
+            // leaves cnt stale and recomputes such vertices every round).
+            // The changed set of every round is the same either way: the
+            // vertices whose count is below their core.  A changed vertex
+            // keeps its stale cnt, which stays a lower bound under the
+            // marking rule whichever of its two values a neighbour reads.
             auto want = [&](uint32_t v) -> bool {
                 if (a.method == HM_CNT)
                     return round == 1 ? a.core[v] > 0 : true; // list holds cnt<core
@@ -751,12 +760,15 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 [&](uint32_t v, uint32_t d) {
                     if (!want(v))
                         return;
-                    uint32_t old = a.core[v];
-                    uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old);
+                    uint32_t old = a.core[v], sum;
+                    uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old, sum);
                     if (lane == 0 && old >= KC_LBINS)
                         n_atom += d; // histogram built in HBM
-                    if (k == old)
+                    if (k == old) {
+                        if (a.method == HM_CNT && lane == 0)
+                            a.cnt[v] = sum; // see cnt_refresh below
                         return;
+                    }
                     if (lane == 0) {
                         a.newc[v] = k;
                         L.stv[wv][n_st] = v;
@@ -776,6 +788,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     bool ch = k != old;
                     if (ch && threadIdx.x == 0)
                         a.newc[v] = k;
+                    if (!ch && a.method == HM_CNT && threadIdx.x == 0)
+                        a.cnt[v] = sum; // see cnt_refresh below
                     if (wv == 0)
                         hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
                     __syncthreads();
