@@ -462,7 +462,8 @@ void kc_graph_free(kc_graph *g)
     }
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
-                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->cs_row[0], &g->cs_row[1],
+                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->hubs, &g->cs_row[0],
+                      &g->cs_row[1],
                       &g->cs_col[0], &g->cs_col[1], &g->cs_deg[0], &g->cs_deg[1], &g->cs_done[0],
                       &g->cs_done[1], &g->cs_gmap[0], &g->cs_gmap[1], &g->cs_loc, &g->cs_wpre,
                       &g->cs_tmp, &g->cs_bad, &g->aux})
@@ -850,6 +851,12 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
     p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
     p.hb_ds = use_ds && g->nnz + g->V < 0xffffffffull ? (const uint32_t *)g->hb_ds.p : nullptr;
+    if (use_ds) { // hub list for the rebuild rounds (at most V entries)
+        if ((s = ensure(g, g->hubs, V4, "hub list")))
+            return s;
+        p.hubs = (uint32_t *)g->hubs.p;
+        p.hub_cap = g->V;
+    }
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;
     p.grid = grid;

@@ -56,6 +56,8 @@ template <typename R> struct HArgs {
     const uint32_t *col_ds; // degree-ordered adjacency (kc_prep.hip) or null
     const uint32_t *nd_ds;  // neighbour degrees beside col_ds
     const uint32_t *hb_ds;  // neighbour histogram bases beside col_ds (nnz+V < 2^32), or null
+    uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null
+    uint32_t hub_cap;
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
@@ -68,6 +70,13 @@ constexpr uint32_t KC_HAGG_LOG2 = 11;
 constexpr uint32_t KC_HAGG = 1u << KC_HAGG_LOG2;
 constexpr uint32_t KC_HAGG_PROBES = 8;
 constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggregated
+// HistoCore rebuild rounds: in a round whose changed vertices reach more than
+// KC_PULL_NUM/KC_PULL_DEN as many adjacency entries as all hubs (vertices of
+// more than KC_PULL_T neighbours) hold, every hub's bins are recounted from its
+// neighbours' values (reads) instead of patched by two atomics per affected
+// edge; updates aimed at the other vertices are patched as usual.
+constexpr uint32_t KC_PULL_T = KC_HUB_AGG;
+constexpr uint64_t KC_PULL_NUM = 1, KC_PULL_DEN = 4;
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
 template <int BLOCK> struct HLds {
@@ -359,6 +368,87 @@ __device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t
     __syncthreads();
 }
 
+// ---------------------------------------------- HistoCore hub rebuilds
+// Recount bins 1..cu of u's histogram (HBM words from rowptr[u]+u) as the
+// histogram of min(core(w), cu) over its neighbours w; bin 0 is kept.
+// Returns the top bin (neighbours at or above cu).  One wave.
+template <typename R, int BLOCK>
+__device__ __forceinline__ uint32_t wave_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u,
+                                                 uint32_t d, uint32_t cu, uint64_t &n_atom)
+{
+    uint32_t lane = lane_id();
+    uint32_t w = threadIdx.x / KC_WAVE;
+    uint32_t *hb = a.histo + hbase(a.rowptr, u);
+    if (cu < KC_LBINS) {
+        wave_hist<R, true>(a, u, cu, a.core, L.bins[w]);
+        for (uint32_t j = 1 + lane; j <= cu; j += KC_WAVE)
+            hb[j] = L.bins[w][j];
+        return L.bins[w][cu];
+    }
+    uint32_t keep = hb[0]; // read by every lane before lane 0 clears it
+    wave_hist<R, false>(a, u, cu, a.core, hb);
+    uint32_t top = ld_relaxed(&hb[cu]);
+    if (lane == 0) {
+        hb[0] = keep;
+        n_atom += d;
+    }
+    return top;
+}
+
+// The same for the whole workgroup: bins in LDS (all waves' bin arrays as one)
+// when cu fits, else in HBM.
+template <typename R, int BLOCK>
+__device__ uint32_t block_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u, uint32_t cu,
+                                  uint64_t &n_atom)
+{
+    constexpr uint32_t NB = (BLOCK / KC_WAVE) * KC_LBINS;
+    uint32_t *hb = a.histo + hbase(a.rowptr, u);
+    uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+    uint32_t top;
+    if (cu < NB) {
+        uint32_t *fb = &L.bins[0][0];
+        for (uint32_t j = threadIdx.x; j <= cu; j += BLOCK)
+            fb[j] = 0;
+        __syncthreads();
+        uint32_t ntop = 0;
+        for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
+            uint32_t x[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint64_t ii = i + (uint64_t)k * BLOCK;
+                x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (x[k] == 0xffffffffu)
+                    continue;
+                if (x[k] >= cu)
+                    ntop++;
+                else
+                    atomicAdd(&fb[x[k]], 1u);
+            }
+        }
+        ntop = wave_sum_u32(ntop);
+        if (lane_id() == 0 && ntop)
+            atomicAdd(&fb[cu], ntop);
+        __syncthreads();
+        for (uint32_t j = 1 + threadIdx.x; j <= cu; j += BLOCK)
+            hb[j] = fb[j];
+        top = fb[cu];
+        __syncthreads(); // the bins are reused by the next item
+        return top;
+    }
+    uint32_t keep = hb[0]; // thread 0 clears bin 0 after its own read
+    block_hist<R, BLOCK>(a, u, cu, hb);
+    top = ld_relaxed(&hb[cu]);
+    __syncthreads(); // every thread has its top before bin 0 is restored
+    if (threadIdx.x == 0) {
+        hb[0] = keep;
+        n_atom += (uint64_t)(e - s);
+    }
+    return top;
+}
+
 // ------------------------------------------------------------ grab loop
 // Runs f(item) for items [0, n) of a list (or the vertex range when lst is
 // null), one item per wave; items with more than KC_HBIG neighbours are
@@ -475,6 +565,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
     //      (histo_kernel, HistoCore.cu:15-75)
     uint32_t n_big = V; // HistoCore: vertices of more than KC_TINY neighbours
+    uint32_t n_hubs = 0;    // HistoCore: listed hubs and their adjacency entries
+    uint64_t hub_edges = 0;
     if (a.method == HM_HISTO) {
         // Most vertices of a power-law graph have a handful of neighbours: a
         // thread builds such a histogram in registers and stores its d+1
@@ -488,6 +580,28 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 bool valid = v < V;
                 uint32_t d = valid ? (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]) : 0u;
                 bool big = valid && d > KC_TINY;
+                // hubs, for the rebuild rounds: count in sum1, edges in sum2
+                bool hub = a.hubs && valid && d > KC_PULL_T;
+                uint64_t hm = ballot(hub);
+                if (hm) {
+                    uint64_t hd = wave_sum_u64(hub ? (uint64_t)d : 0ull);
+                    uint32_t leader = __ffsll((unsigned long long)hm) - 1;
+                    uint32_t hb = 0;
+                    if (lane == leader) {
+                        hb = (uint32_t)atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum1,
+                                                 (unsigned long long)__popcll(hm));
+                        atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum2,
+                                  (unsigned long long)hd);
+                    }
+                    hb = __shfl(hb, leader, KC_WAVE);
+                    if (hub) {
+                        uint32_t p = hb + lane_rank(hm);
+                        if (p < a.hub_cap)
+                            a.hubs[p] = v;
+                        else
+                            sync_fail(gs, 2);
+                    }
+                }
                 if (valid && !big) {
                     uint64_t s0 = a.rowptr[v], b = s0 + v;
                     uint32_t x[KC_TINY];
@@ -518,6 +632,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         if (!hbar(a, gs, gen, round, 0))
             goto out;
         n_big = ld_relaxed(&slot_of(gs, phase)[0].qtail);
+        n_hubs = (uint32_t)ld_relaxed64(&slot_of(gs, phase)[0].sum1);
+        hub_edges = ld_relaxed64(&slot_of(gs, phase)[0].sum2);
         phase = gen;
         for_items<BLOCK, R>(
             L, a, phase, n_big, a.list[3],
@@ -831,6 +947,59 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             // nodes of the changed vertices: each wave takes KC_HGRAB nodes of
             // KC_HCH edges from the shared counter
             uint32_t *f_list = a.list[0];
+            const bool pull = n_hubs && a.nd_ds &&
+                              (uint64_t)n_chunks * KC_HCH * KC_PULL_DEN >
+                                  hub_edges * KC_PULL_NUM;
+            if (pull) {
+                // Rebuild every hub's bins 1..core (bin 0 keeps the value
+                // the last collapse stored; bins above core are left as
+                // they are).  After a patch round bin j < core counts the
+                // neighbours of value j and bin core those at or above it,
+                // and the top bin was >= core when the phase began, so the
+                // recount is word for word the patched histogram and the
+                // hub is flagged exactly when its top bin ends below core.
+                // Hubs go first, so the edge walk below fills the tail.
+                if (threadIdx.x == 0)
+                    L.nbig = 0;
+                __syncthreads();
+                for (;;) {
+                    uint32_t base = 0;
+                    if (lane == 0)
+                        base = atomicAdd(&slot_of(gs, phase)[0].flag, KC_HGRAB);
+                    base = __shfl(base, 0, KC_WAVE);
+                    if (base >= n_hubs)
+                        break;
+                    uint32_t end = base + KC_HGRAB < n_hubs ? base + KC_HGRAB : n_hubs;
+                    for (uint32_t it = base; it < end; it++) {
+                        uint32_t u = a.hubs[it];
+                        uint32_t d = (uint32_t)(a.rowptr[u + 1] - a.rowptr[u]);
+                        uint32_t cu = a.core[u];
+                        if (d > KC_HBIG || cu >= KC_LBINS) { // to the workgroup
+                            uint32_t slot = 0;
+                            if (lane == 0)
+                                slot = atomicAdd(&L.nbig, 1u);
+                            slot = __shfl(slot, 0, KC_WAVE);
+                            if (slot < KC_BIGQ) {
+                                if (lane == 0)
+                                    L.bigq[slot] = u;
+                                continue;
+                            }
+                        }
+                        uint32_t top = wave_rebuild<R, BLOCK>(L, a, u, d, cu, n_atom);
+                        hpush(gs, phase, f_list, cap_list, lane == 0 && top < cu, u);
+                    }
+                }
+                __syncthreads();
+                uint32_t nbig = L.nbig < KC_BIGQ ? L.nbig : KC_BIGQ;
+                for (uint32_t i = 0; i < nbig; i++) {
+                    uint32_t u = L.bigq[i];
+                    uint32_t cu = a.core[u];
+                    uint32_t top = block_rebuild<R, BLOCK>(L, a, u, cu, n_atom);
+                    if (wv == 0)
+                        hpush(gs, phase, f_list, cap_list, lane == 0 && top < cu, u);
+                    __syncthreads();
+                }
+            }
             for (;;) {
                 uint32_t base = 0;
                 if (lane == 0)
@@ -853,7 +1022,8 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                         uint64_t i = i0 + lane;
                         bool flag = false;
                         uint32_t u = 0;
-                        if (i < e) {
+                        // a rebuild round leaves hubs to the pass below
+                        if (i < e && !(pull && a.nd_ds[i] > KC_PULL_T)) {
                             u = adj[i];
                             uint64_t bu = a.hb_ds ? (uint64_t)a.hb_ds[i] : 0;
                             bool hub = a.hb_ds && a.nd_ds[i] > KC_HUB_AGG;
@@ -1149,6 +1319,8 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     a.col_ds = p.col_ds;
     a.nd_ds = p.nd_ds;
     a.hb_ds = p.hb_ds;
+    a.hubs = p.method == HM_HISTO ? p.hubs : nullptr;
+    a.hub_cap = p.hub_cap;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;

@@ -127,3 +127,30 @@ def test_atomic_count_bounds(ctx, method):
         # update-walk entry at most a bin decrement and a bin increment, or
         # their summed forms
         assert st["atomics"] <= hg.nnz + 2 * st["edges_walked"]
+
+
+def _clique_with_leaves(n, leaves):
+    """K_n whose members carry `leaves` pendant vertices each: every clique
+    member starts at degree n-1+leaves and drops to n-1 in the first round."""
+    iu, ju = np.triu_indices(n, 1)
+    owner = np.repeat(np.arange(n), leaves)
+    leaf = n + np.arange(n * leaves)
+    a = np.concatenate([iu, owner]).astype(np.int64)
+    b = np.concatenate([ju, leaf]).astype(np.int64)
+    src, dst = np.concatenate([a, b]), np.concatenate([b, a])
+    order = np.lexsort((dst, src))
+    V = n + n * leaves
+    rowptr = np.zeros(V + 1, np.uint64)
+    rowptr[1:] = np.cumsum(np.bincount(src, minlength=V))
+    return rowptr, dst[order].astype(np.uint32)
+
+
+@pytest.mark.parametrize("n,block,grid", [(600, 512, 0), (1000, 512, 3), (2100, 256, 0)])
+def test_histocore_hub_rebuild_paths(ctx, n, block, grid):
+    # a round that reaches many hub edges recounts every hub's bins instead
+    # of patching them: cores of 599 / 999 go to the workgroup (LDS bins), a
+    # workgroup's overflow of such hubs (grid 3) to one wave with HBM bins,
+    # and 2099 (above a 256-thread group's LDS bins) to HBM bins; bin 0 and
+    # the bins above the core must come out as the reference leaves them
+    rp, col = _clique_with_leaves(n, 10)
+    _check(ctx, rp, col, "histocore", hist=True, block=block, grid=grid)
