@@ -39,6 +39,8 @@ constexpr uint32_t KC_TINY = 16;
 constexpr uint32_t KC_HGRAB = 16;   // items taken per wavefront grab
 constexpr uint32_t KC_BIGQ = 256;   // per-WG queue of big items
 constexpr uint32_t KC_HCH = 64;     // edges per virtual node of a changed vertex
+static_assert(KC_HCH == KC_WAVE, "a virtual node is one wave-wide pass");
+constexpr uint32_t KC_WALK_U = 2;   // virtual nodes in flight per wave in the update walk
 
 template <typename R> struct HArgs {
     const R *rowptr;
@@ -1000,6 +1002,28 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     __syncthreads();
                 }
             }
+            // One affected edge (v: oc -> nc, neighbour u of value cu > nc):
+            // patch u's bins, summed per workgroup for hubs and applied when
+            // the phase ends (flush below).  Returns whether this decrement
+            // carried u's top bin below its core.
+            auto patch = [&](uint32_t u, uint64_t bu, bool hub, uint32_t cu, uint32_t nc,
+                             uint32_t oc) -> bool {
+                bool flag = false;
+                if (!a.hb_ds)
+                    bu = hbase(a.rowptr, u);
+                uint32_t sub = cu < oc ? cu : oc;
+                if (!(hub && hagg_add(L, (uint32_t)(bu + sub), -1, u, sub == cu ? cu : KC_HEMPTY))) {
+                    uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
+                    flag = oc >= cu && c == cu;
+                    n_atom++;
+                }
+                if (!(hub && hagg_add(L, (uint32_t)(bu + nc), 1, u, KC_HEMPTY))) {
+                    atomicAdd(&a.histo[bu + nc], 1u);
+                    n_atom++;
+                }
+                return flag;
+            };
+            const uint32_t *adj = a.col_ds ? a.col_ds : a.col;
             for (;;) {
                 uint32_t base = 0;
                 if (lane == 0)
@@ -1007,47 +1031,60 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                 base = __shfl(base, 0, KC_WAVE);
                 if (base >= n_chunks)
                     break;
-                uint32_t end = base + KC_HGRAB < n_chunks ? base + KC_HGRAB : n_chunks;
-                for (uint32_t it = base; it < end; it++) {
-                    uint64_t item = a.chunks[it];
+                const uint32_t cnt = base + KC_HGRAB < n_chunks ? KC_HGRAB : n_chunks - base;
+                // the grab's virtual nodes, one per lane (lanes past cnt hold
+                // an empty range): edge range and the vertex's new / old value
+                uint64_t m_s = 0, m_e = 0;
+                uint32_t m_nc = 0, m_oc = 0;
+                if (lane < cnt) {
+                    uint64_t item = a.chunks[base + lane];
                     uint32_t v = (uint32_t)item;
                     uint64_t rs = a.rowptr[v], re = a.rowptr[v + 1];
-                    uint64_t s0 = rs + (uint64_t)(uint32_t)(item >> 32) * KC_HCH;
-                    uint64_t e = s0 + KC_HCH < re ? s0 + KC_HCH : re;
-                    uint32_t nc = a.core[v], oc = a.histo[rs + v];
-                    if (lane == 0)
-                        upd_edges += e - s0;
-                    const uint32_t *adj = a.col_ds ? a.col_ds : a.col;
-                    for (uint64_t i0 = s0; i0 < e; i0 += KC_WAVE) {
-                        uint64_t i = i0 + lane;
-                        bool flag = false;
-                        uint32_t u = 0;
-                        // a rebuild round leaves hubs to the pass below
-                        if (i < e && !(pull && a.nd_ds[i] > KC_PULL_T)) {
-                            u = adj[i];
-                            uint64_t bu = a.hb_ds ? (uint64_t)a.hb_ds[i] : 0;
-                            bool hub = a.hb_ds && a.nd_ds[i] > KC_HUB_AGG;
-                            uint32_t cu = a.core[u];
-                            if (nc < cu) {
-                                if (!a.hb_ds)
-                                    bu = hbase(a.rowptr, u);
-                                uint32_t sub = cu < oc ? cu : oc;
-                                // a hub's updates are summed per workgroup and
-                                // applied when the phase ends (flush below)
-                                if (!(hub && hagg_add(L, (uint32_t)(bu + sub), -1, u,
-                                                      sub == cu ? cu : KC_HEMPTY))) {
-                                    uint32_t c = atomicSub(&a.histo[bu + sub], 1u);
-                                    flag = oc >= cu && c == cu;
-                                    n_atom++;
-                                }
-                                if (!(hub && hagg_add(L, (uint32_t)(bu + nc), 1, u, KC_HEMPTY))) {
-                                    atomicAdd(&a.histo[bu + nc], 1u);
-                                    n_atom++;
-                                }
-                            }
+                    m_s = rs + (uint64_t)(uint32_t)(item >> 32) * KC_HCH;
+                    m_e = m_s + KC_HCH < re ? m_s + KC_HCH : re;
+                    m_nc = a.core[v];
+                    m_oc = a.histo[rs + v];
+                    upd_edges += m_e - m_s;
+                }
+                // KC_WALK_U virtual nodes (one wave-wide pass each) in flight
+                for (uint32_t k = 0; k < cnt; k += KC_WALK_U) {
+                    uint64_t i[KC_WALK_U], bu[KC_WALK_U];
+                    uint32_t nc[KC_WALK_U], oc[KC_WALK_U], u[KC_WALK_U], nd[KC_WALK_U];
+                    uint32_t cu[KC_WALK_U];
+                    bool in[KC_WALK_U], f[KC_WALK_U];
+#pragma unroll
+                    for (uint32_t j = 0; j < KC_WALK_U; j++) {
+                        i[j] = __shfl(m_s, (int)(k + j), KC_WAVE) + lane;
+                        in[j] = i[j] < __shfl(m_e, (int)(k + j), KC_WAVE);
+                        nc[j] = __shfl(m_nc, (int)(k + j), KC_WAVE);
+                        oc[j] = __shfl(m_oc, (int)(k + j), KC_WAVE);
+                        u[j] = 0;
+                        nd[j] = 0;
+                        bu[j] = 0;
+                        if (in[j]) {
+                            u[j] = adj[i[j]];
+                            if (a.hb_ds)
+                                bu[j] = a.hb_ds[i[j]];
+                            if (a.nd_ds)
+                                nd[j] = a.nd_ds[i[j]];
                         }
-                        hpush(gs, phase, f_list, cap_list, flag, u);
                     }
+#pragma unroll
+                    for (uint32_t j = 0; j < KC_WALK_U; j++) {
+                        if (pull && nd[j] > KC_PULL_T)
+                            in[j] = false; // a rebuild round's hubs are done above
+                        cu[j] = in[j] ? a.core[u[j]] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < KC_WALK_U; j++) {
+                        f[j] = false;
+                        if (in[j] && nc[j] < cu[j])
+                            f[j] = patch(u[j], bu[j], a.hb_ds && nd[j] > KC_HUB_AGG, cu[j], nc[j],
+                                         oc[j]);
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < KC_WALK_U; j++)
+                        hpush(gs, phase, f_list, cap_list, f[j], u[j]);
                 }
             }
             // Apply the summed hub updates.  A top bin only ever decreases in
