@@ -769,7 +769,8 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     uint32_t block = o.block ? o.block : 512;
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
-    int occ = kc::hindex_occupancy(block, g->wide);
+    const uint32_t mi = method == KC_HISTO ? 0 : method == KC_CNT ? 1 : 2;
+    int occ = kc::hindex_occupancy(block, g->wide, mi);
     if (occ <= 0)
         return fail(KC_EDEVICE, "h-index kernel cannot be resident with block %u", block);
     uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
@@ -832,7 +833,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.rowptr = g->rowptr;
     p.col = g->col;
     p.V = g->V;
-    p.method = method == KC_HISTO ? 0 : method == KC_CNT ? 1 : 2;
+    p.method = mi;
     p.core = (uint32_t *)g->deg.p;
     p.histo = (uint32_t *)g->histo.p;
     p.cnt = (uint32_t *)g->cnt.p;

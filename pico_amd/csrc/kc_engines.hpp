@@ -146,7 +146,8 @@ struct HindexLaunch {
 };
 
 hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st);
-int hindex_occupancy(uint32_t block, bool wide);
+// method: 0 HistoCore, 1 CntCore, 2 NbrCore (one kernel instance each)
+int hindex_occupancy(uint32_t block, bool wide, uint32_t method);
 
 size_t grid_sync_bytes();
 

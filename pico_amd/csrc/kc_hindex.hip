@@ -228,11 +228,127 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
     __threadfence_block();
 }
 
+// Histogram of min(cap, core(u)) over u in N(v) into HBM bins[1..cap] for
+// cap >= KC_LBINS, by one wave: values below KC_LBINS are counted in the
+// wave's LDS bins and stored once, only larger ones (few: neighbours of high
+// core) become HBM atomics.  bins[0] is not written.
+template <typename R, int BLOCK>
+__device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
+                              uint32_t *bins, uint64_t &n_atom)
+{
+    uint32_t lane = lane_id();
+    uint32_t *lb = L.bins[threadIdx.x / KC_WAVE];
+    for (uint32_t j = lane; j < KC_LBINS; j += KC_WAVE)
+        lb[j] = 0;
+    for (uint32_t j = KC_LBINS + lane; j <= cap; j += KC_WAVE)
+        bins[j] = 0;
+    vm_drain(); // the clears land before any atomic on those words
+    __threadfence_block();
+    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+    uint32_t ntop = 0, nhbm = 0;
+    for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t ii = i + (uint64_t)k * KC_WAVE;
+            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (x[k] == 0xffffffffu)
+                continue;
+            if (x[k] >= cap) {
+                ntop++;
+            } else if (x[k] < KC_LBINS) {
+                atomicAdd(&lb[x[k]], 1u);
+            } else {
+                atomicAdd(&bins[x[k]], 1u);
+                nhbm++;
+            }
+        }
+    }
+    ntop = wave_sum_u32(ntop);
+    nhbm = wave_sum_u32(nhbm);
+    if (lane == 0) {
+        if (ntop)
+            atomicAdd(&bins[cap], ntop);
+        n_atom += nhbm + (ntop ? 1 : 0);
+    }
+    __threadfence_block();
+    for (uint32_t j = 1 + lane; j < KC_LBINS; j += KC_WAVE)
+        bins[j] = lb[j];
+    vm_drain(); // read back by this wave next
+    __threadfence_block();
+}
+
+// The same by the whole workgroup, its LDS bins being all waves' arrays as
+// one (NB words); values from NB up go to HBM atomics.  Returns bins[cap].
+template <typename R, int BLOCK>
+__device__ uint32_t block_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
+                                   uint32_t *bins, uint64_t &n_atom)
+{
+    constexpr uint32_t NB = (BLOCK / KC_WAVE) * KC_LBINS;
+    uint32_t *fb = &L.bins[0][0];
+    const bool fits = cap < NB;
+    const uint32_t lim = fits ? cap + 1 : NB; // LDS words in use
+    for (uint32_t j = threadIdx.x; j < lim; j += BLOCK)
+        fb[j] = 0;
+    for (uint32_t j = NB + threadIdx.x; j <= cap; j += BLOCK)
+        bins[j] = 0;
+    vm_drain();
+    __syncthreads();
+    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+    uint32_t ntop = 0, nhbm = 0;
+    for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t ii = i + (uint64_t)k * BLOCK;
+            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (x[k] == 0xffffffffu)
+                continue;
+            if (x[k] >= cap) {
+                ntop++;
+            } else if (x[k] < NB) {
+                atomicAdd(&fb[x[k]], 1u);
+            } else {
+                atomicAdd(&bins[x[k]], 1u);
+                nhbm++;
+            }
+        }
+    }
+    ntop = wave_sum_u32(ntop);
+    nhbm = wave_sum_u32(nhbm);
+    if (lane_id() == 0) {
+        if (ntop) {
+            if (fits)
+                atomicAdd(&fb[cap], ntop);
+            else
+                atomicAdd(&bins[cap], ntop);
+        }
+        n_atom += nhbm + (ntop && !fits ? 1 : 0);
+    }
+    vm_drain();
+    __syncthreads();
+    const uint32_t hi = fits ? cap : NB - 1;
+    for (uint32_t j = 1 + threadIdx.x; j <= hi; j += BLOCK)
+        bins[j] = fb[j];
+    uint32_t top = fits ? fb[cap] : 0u;
+    vm_drain(); // the bins are read back (device-coherent loads) after this
+    __syncthreads();
+    if (!fits)
+        top = ld_relaxed(&bins[cap]);
+    return top;
+}
+
 // h-index k of v's neighbour values capped at cap, computed by one wave, and
 // the number of neighbours whose value is at least k.
 template <typename R, int BLOCK>
 __device__ __forceinline__ uint32_t wave_hindex(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v,
-                                                uint32_t cap, uint32_t &sum)
+                                                uint32_t cap, uint32_t &sum, uint64_t &n_atom)
 {
     sum = 0;
     if (cap == 0)
@@ -243,7 +359,7 @@ __device__ __forceinline__ uint32_t wave_hindex(HLds<BLOCK> &L, const HArgs<R> &
         return wave_hsearch(L.bins[w], cap, sum);
     }
     uint32_t *gb = a.histo + hbase(a.rowptr, v);
-    wave_hist<R, false>(a, v, cap, a.core, gb);
+    wave_hist_hyb<R, BLOCK>(L, a, v, cap, gb, n_atom);
     return wave_hsearch(gb, cap, sum);
 }
 
@@ -333,43 +449,6 @@ __device__ uint32_t block_hsearch(HLds<BLOCK> &L, uint32_t *bins, uint32_t top,
     return 0;
 }
 
-// Block-wide histogram of min(cap, val(u)) into HBM bins[0..cap].
-template <typename R, int BLOCK>
-__device__ void block_hist(const HArgs<R> &a, uint32_t v, uint32_t cap, uint32_t *bins)
-{
-    // Only this workgroup touches these bins, and they are read back with
-    // device-coherent loads: completing the stores / atomics (vmcnt) orders
-    // them without an L2-flushing fence.
-    for (uint32_t j = threadIdx.x; j <= cap; j += BLOCK)
-        st_relaxed(&bins[j], 0u);
-    vm_drain();
-    __syncthreads();
-    uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
-    uint32_t ntop = 0; // values >= cap: one top-bin addition per wave (see wave_hist)
-    for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
-        uint32_t x[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            uint64_t ii = i + (uint64_t)k * BLOCK;
-            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (x[k] == 0xffffffffu)
-                continue;
-            if (x[k] >= cap)
-                ntop++;
-            else
-                atomicAdd(&bins[x[k]], 1u);
-        }
-    }
-    ntop = wave_sum_u32(ntop);
-    if (lane_id() == 0 && ntop)
-        atomicAdd(&bins[cap], ntop);
-    vm_drain();
-    __syncthreads();
-}
-
 // ---------------------------------------------- HistoCore hub rebuilds
 // Recount bins 1..cu of u's histogram (HBM words from rowptr[u]+u) as the
 // histogram of min(core(w), cu) over its neighbours w; bin 0 is kept.
@@ -387,68 +466,16 @@ __device__ __forceinline__ uint32_t wave_rebuild(HLds<BLOCK> &L, const HArgs<R> 
             hb[j] = L.bins[w][j];
         return L.bins[w][cu];
     }
-    uint32_t keep = hb[0]; // read by every lane before lane 0 clears it
-    wave_hist<R, false>(a, u, cu, a.core, hb);
-    uint32_t top = ld_relaxed(&hb[cu]);
-    if (lane == 0) {
-        hb[0] = keep;
-        n_atom += d;
-    }
-    return top;
+    wave_hist_hyb<R, BLOCK>(L, a, u, cu, hb, n_atom);
+    return ld_relaxed(&hb[cu]);
 }
 
-// The same for the whole workgroup: bins in LDS (all waves' bin arrays as one)
-// when cu fits, else in HBM.
+// The same for the whole workgroup (LDS bins of all its waves as one).
 template <typename R, int BLOCK>
 __device__ uint32_t block_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u, uint32_t cu,
                                   uint64_t &n_atom)
 {
-    constexpr uint32_t NB = (BLOCK / KC_WAVE) * KC_LBINS;
-    uint32_t *hb = a.histo + hbase(a.rowptr, u);
-    uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
-    uint32_t top;
-    if (cu < NB) {
-        uint32_t *fb = &L.bins[0][0];
-        for (uint32_t j = threadIdx.x; j <= cu; j += BLOCK)
-            fb[j] = 0;
-        __syncthreads();
-        uint32_t ntop = 0;
-        for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
-            uint32_t x[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint64_t ii = i + (uint64_t)k * BLOCK;
-                x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (x[k] == 0xffffffffu)
-                    continue;
-                if (x[k] >= cu)
-                    ntop++;
-                else
-                    atomicAdd(&fb[x[k]], 1u);
-            }
-        }
-        ntop = wave_sum_u32(ntop);
-        if (lane_id() == 0 && ntop)
-            atomicAdd(&fb[cu], ntop);
-        __syncthreads();
-        for (uint32_t j = 1 + threadIdx.x; j <= cu; j += BLOCK)
-            hb[j] = fb[j];
-        top = fb[cu];
-        __syncthreads(); // the bins are reused by the next item
-        return top;
-    }
-    uint32_t keep = hb[0]; // thread 0 clears bin 0 after its own read
-    block_hist<R, BLOCK>(a, u, cu, hb);
-    top = ld_relaxed(&hb[cu]);
-    __syncthreads(); // every thread has its top before bin 0 is restored
-    if (threadIdx.x == 0) {
-        hb[0] = keep;
-        n_atom += (uint64_t)(e - s);
-    }
-    return top;
+    return block_hist_hyb<R, BLOCK>(L, a, u, cu, a.histo + hbase(a.rowptr, u), n_atom);
 }
 
 // ------------------------------------------------------------ grab loop
@@ -538,7 +565,8 @@ __device__ __forceinline__ bool hbar(const HArgs<R> &a, GridSync *gs, uint32_t &
 }
 
 // ---------------------------------------------------------------- kernel
-template <int BLOCK, typename R>
+// One instance per method (M), so each carries only its own paths' registers.
+template <int BLOCK, typename R, uint32_t M>
 __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 {
     __shared__ HLds<BLOCK> L;
@@ -569,7 +597,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
     uint32_t n_big = V; // HistoCore: vertices of more than KC_TINY neighbours
     uint32_t n_hubs = 0;    // HistoCore: listed hubs and their adjacency entries
     uint64_t hub_edges = 0;
-    if (a.method == HM_HISTO) {
+    if (M == HM_HISTO) {
         // Most vertices of a power-law graph have a handful of neighbours: a
         // thread builds such a histogram in registers and stores its d+1
         // words; the others are listed for the wave / workgroup builders.
@@ -651,21 +679,17 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     for (uint32_t j = lane; j <= d; j += KC_WAVE)
                         a.histo[b + j] = j == 0 ? d : L.bins[wv][j];
                 } else {
-                    wave_hist<R, false>(a, v, d, a.core, a.histo + b);
-                    if (lane == 0) {
+                    wave_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+                    if (lane == 0)
                         a.histo[b] = d;
-                        n_atom += d;
-                    }
                 }
             },
             [&](uint32_t v) {
                 uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
                 uint64_t b = hbase(a.rowptr, v);
-                block_hist<R, BLOCK>(a, v, d, a.histo + b);
-                if (threadIdx.x == 0) {
+                block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+                if (threadIdx.x == 0)
                     a.histo[b] = d;
-                    n_atom += d;
-                }
                 __syncthreads();
             });
         if (!hbar(a, gs, gen, round, 0))
@@ -681,15 +705,15 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         uint32_t *c_list = a.list[1];
         if (round == 1) {
             n_in = V; // every vertex; the per-method filter is applied below
-            if (a.method == HM_HISTO) { // tiny vertices first, thread by thread
+            if (M == HM_HISTO) { // tiny vertices first, thread by thread
                 n_in = n_big;
                 in_list = a.list[3];
             }
         } else {
             n_in = L.done; // length of the list the previous phase built
-            in_list = a.method == HM_CNT ? act_cur : a.list[0];
+            in_list = M == HM_CNT ? act_cur : a.list[0];
         }
-        if (a.method == HM_HISTO) {
+        if (M == HM_HISTO) {
             // count_kernel (HistoCore.cu:77-128): scan bins from the old core
             // down; collapse on change
             // A changed vertex is queued as ceil(deg/KC_HCH) wavefront-sized
@@ -868,7 +892,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             // keeps its stale cnt, which stays a lower bound under the
             // marking rule whichever of its two values a neighbour reads.
             auto want = [&](uint32_t v) -> bool {
-                if (a.method == HM_CNT)
+                if (M == HM_CNT)
                     return round == 1 ? a.core[v] > 0 : true; // list holds cnt<core
                 return true;
             };
@@ -879,11 +903,9 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     if (!want(v))
                         return;
                     uint32_t old = a.core[v], sum;
-                    uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old, sum);
-                    if (lane == 0 && old >= KC_LBINS)
-                        n_atom += d; // histogram built in HBM
+                    uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old, sum, n_atom);
                     if (k == old) {
-                        if (a.method == HM_CNT && lane == 0)
+                        if (M == HM_CNT && lane == 0)
                             a.cnt[v] = sum; // see cnt_refresh below
                         return;
                     }
@@ -899,14 +921,12 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                     uint32_t old = a.core[v];
                     uint32_t sum;
                     uint64_t b = hbase(a.rowptr, v);
-                    block_hist<R, BLOCK>(a, v, old, a.histo + b);
-                    if (threadIdx.x == 0)
-                        n_atom += (uint64_t)(a.rowptr[v + 1] - a.rowptr[v]);
+                    block_hist_hyb<R, BLOCK>(L, a, v, old, a.histo + b, n_atom);
                     uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
                     bool ch = k != old;
                     if (ch && threadIdx.x == 0)
                         a.newc[v] = k;
-                    if (!ch && a.method == HM_CNT && threadIdx.x == 0)
+                    if (!ch && M == HM_CNT && threadIdx.x == 0)
                         a.cnt[v] = sum; // see cnt_refresh below
                     if (wv == 0)
                         hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
@@ -934,7 +954,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             goto out;
         // HistoCore counts changed vertices in sum0 and queues their virtual
         // nodes on the chunk list; the others list changed vertices
-        uint32_t n_changed = a.method == HM_HISTO
+        uint32_t n_changed = M == HM_HISTO
                                  ? (uint32_t)ld_relaxed64(&slot_of(gs, phase)[0].sum0)
                                  : ld_relaxed(&slot_of(gs, phase)[0].qtail);
         uint32_t n_chunks = ld_relaxed(&slot_of(gs, phase)[0].qtail);
@@ -944,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
 
         // ------------------------------------------------ update phase
         phase = gen;
-        if (a.method == HM_HISTO) {
+        if (M == HM_HISTO) {
             // update_histo_kernel (HistoCore.cu:130-202) over the virtual
             // nodes of the changed vertices: each wave takes KC_HGRAB nodes of
             // KC_HCH edges from the shared counter
@@ -1124,7 +1144,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
         // NbrCore / CntCore: apply the deferred values and mark neighbours
         // (nbr_kernel, NbrCore.cu:115-167 / CntCore.cu:132-195)
         {
-            uint32_t *m_list = a.method == HM_NBR ? a.list[0] : a.list[2];
+            uint32_t *m_list = M == HM_NBR ? a.list[0] : a.list[2];
             uint32_t epoch = round;
             // Which neighbours u of v (core ov -> nc) to mark.  core[u] may be
             // read before or after u's own update of this phase; both choices
@@ -1138,7 +1158,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             // cnt < core stays active and each round changes the same set.
             auto affects = [&](uint32_t u, uint32_t nc, uint32_t ov) -> bool {
                 uint32_t cu = a.core[u];
-                return nc < cu && (a.method == HM_NBR || cu <= ov);
+                return nc < cu && (M == HM_NBR || cu <= ov);
             };
             for_items<BLOCK, R>(
                 L, a, phase, n_changed, c_list,
@@ -1164,7 +1184,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                 a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
-                                if (fresh[k] && a.method == HM_CNT)
+                                if (fresh[k] && M == HM_CNT)
                                     a.cnt[u[k]] = 0;
                             }
                         }
@@ -1194,7 +1214,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
                                 a.mark[u[k]] != epoch) {
                                 fresh[k] = atomicMax(&a.mark[u[k]], epoch) != epoch;
                                 n_atom++;
-                                if (fresh[k] && a.method == HM_CNT)
+                                if (fresh[k] && M == HM_CNT)
                                     a.cnt[u[k]] = 0;
                             }
                         }
@@ -1207,7 +1227,7 @@ __global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
             if (!hbar(a, gs, gen, round, 2))
                 goto out;
             uint32_t n_marked = ld_relaxed(&slot_of(gs, phase)[0].qtail);
-            if (a.method == HM_NBR) {
+            if (M == HM_NBR) {
                 if (threadIdx.x == 0)
                     L.done = n_marked;
                 __syncthreads();
@@ -1370,33 +1390,44 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     if (e != hipSuccess)
         return e;
     void *args[] = {&a};
-    e = launch_persistent((const void *)hindex_kernel<BLOCK, R>, dim3(p.grid), dim3(BLOCK),
-                          args, st);
+    const void *fn = p.method == HM_HISTO ? (const void *)hindex_kernel<BLOCK, R, HM_HISTO>
+                     : p.method == HM_CNT ? (const void *)hindex_kernel<BLOCK, R, HM_CNT>
+                                          : (const void *)hindex_kernel<BLOCK, R, HM_NBR>;
+    e = launch_persistent(fn, dim3(p.grid), dim3(BLOCK), args, st);
     return e;
 }
 
-template <int BLOCK> static int hocc_t(bool wide)
+template <int BLOCK, typename R> static int hocc_t(uint32_t method)
 {
     int nb = 0;
-    hipError_t e = wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &nb, hindex_kernel<BLOCK, uint64_t>, BLOCK, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &nb, hindex_kernel<BLOCK, uint32_t>, BLOCK, 0);
+    hipError_t e =
+        method == HM_HISTO
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hindex_kernel<BLOCK, R, HM_HISTO>,
+                                                           BLOCK, 0)
+        : method == HM_CNT
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hindex_kernel<BLOCK, R, HM_CNT>,
+                                                           BLOCK, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hindex_kernel<BLOCK, R, HM_NBR>,
+                                                           BLOCK, 0);
     return e == hipSuccess ? nb : 0;
 }
 
-int hindex_occupancy(uint32_t block, bool wide)
+int hindex_occupancy(uint32_t block, bool wide, uint32_t method)
 {
+    if (method > HM_NBR)
+        return 0;
     switch (block) {
-    case 256: return hocc_t<256>(wide);
-    case 512: return hocc_t<512>(wide);
-    case 1024: return hocc_t<1024>(wide);
+    case 256: return wide ? hocc_t<256, uint64_t>(method) : hocc_t<256, uint32_t>(method);
+    case 512: return wide ? hocc_t<512, uint64_t>(method) : hocc_t<512, uint32_t>(method);
+    case 1024: return wide ? hocc_t<1024, uint64_t>(method) : hocc_t<1024, uint32_t>(method);
     default: return 0;
     }
 }
 
 hipError_t launch_hindex(const HindexLaunch &p, hipStream_t st)
 {
+    if (p.method > HM_NBR)
+        return hipErrorInvalidValue;
     switch (p.block) {
     case 256: return p.wide ? launch_hindex_t<256, uint64_t>(p, st) : launch_hindex_t<256, uint32_t>(p, st);
     case 512: return p.wide ? launch_hindex_t<512, uint64_t>(p, st) : launch_hindex_t<512, uint32_t>(p, st);
