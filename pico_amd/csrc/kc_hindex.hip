@@ -567,7 +567,7 @@ __device__ __forceinline__ bool hbar(const HArgs<R> &a, GridSync *gs, uint32_t &
 // ---------------------------------------------------------------- kernel
 // One instance per method (M), so each carries only its own paths' registers.
 template <int BLOCK, typename R, uint32_t M>
-__global__ __launch_bounds__(BLOCK) void hindex_kernel(HArgs<R> a)
+__global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HArgs<R> a)
 {
     __shared__ HLds<BLOCK> L;
     GridSync *gs = a.gs;
