@@ -105,11 +105,10 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, uint32_t src)
 }
 
 template <typename R, typename R2>
-__global__ __launch_bounds__(CB) void copy_short(const R *rowptr, const uint32_t *col,
-                                                 const LiveWord *lw, const uint32_t *wpre,
-                                                 const uint32_t *loc, uint32_t Vn, uint32_t nw,
-                                                 const R2 *orow, uint32_t *ocol, uint64_t cap,
-                                                 uint32_t *bad)
+__device__ void copy_short(uint32_t bid, uint32_t nblocks, const R *rowptr, const uint32_t *col,
+                           const LiveWord *lw, const uint32_t *wpre, const uint32_t *loc,
+                           uint32_t Vn, uint32_t nw, const R2 *orow, uint32_t *ocol, uint64_t cap,
+                           uint32_t *bad)
 {
     constexpr uint32_t U = 4;
     const uint32_t lane = lane_id();
@@ -118,8 +117,8 @@ __global__ __launch_bounds__(CB) void copy_short(const R *rowptr, const uint32_t
             atomicOr(bad, 16u);
         return;
     }
-    const uint32_t nwaves = gridDim.x * (CB / KC_WAVE);
-    for (uint32_t i0 = (blockIdx.x * (CB / KC_WAVE) + threadIdx.x / KC_WAVE) * SR; i0 < Vn;
+    const uint32_t nwaves = nblocks * (CB / KC_WAVE);
+    for (uint32_t i0 = (bid * (CB / KC_WAVE) + threadIdx.x / KC_WAVE) * SR; i0 < Vn;
          i0 += nwaves * SR) {
         uint32_t il = i0 + lane;
         uint64_t ls = 0, le = 0, lob = 0, loe = 0;
@@ -168,11 +167,10 @@ __global__ __launch_bounds__(CB) void copy_short(const R *rowptr, const uint32_t
 
 // One workgroup per long row, 16 entries per thread per step.
 template <typename R, typename R2>
-__global__ __launch_bounds__(CB) void copy_long(const R *rowptr, const uint32_t *col,
-                                                const LiveWord *lw, const uint32_t *wpre,
-                                                const uint32_t *loc, const uint32_t *longs,
-                                                uint32_t Vn, uint32_t nw, const R2 *orow,
-                                                uint32_t *ocol, uint64_t cap, uint32_t *bad)
+__device__ void copy_long(uint32_t bid, uint32_t nblocks, const R *rowptr, const uint32_t *col,
+                          const LiveWord *lw, const uint32_t *wpre, const uint32_t *loc,
+                          const uint32_t *longs, uint32_t Vn, uint32_t nw, const R2 *orow,
+                          uint32_t *ocol, uint64_t cap, uint32_t *bad)
 {
     if (wpre[nw] != Vn)
         return; // reported by copy_short
@@ -182,7 +180,7 @@ __global__ __launch_bounds__(CB) void copy_long(const R *rowptr, const uint32_t 
     __shared__ uint32_t s_w[NW * U];
     __shared__ uint32_t s_pre[NW * U + 1];
     const uint32_t lane = lane_id(), w = threadIdx.x / KC_WAVE;
-    for (uint32_t li = blockIdx.x; li < nlong; li += gridDim.x) {
+    for (uint32_t li = bid; li < nlong; li += nblocks) {
         uint32_t i = longs[li];
         uint32_t v = loc[i];
         uint64_t s = (uint64_t)rowptr[v], e = (uint64_t)rowptr[v + 1];
@@ -233,6 +231,23 @@ __global__ __launch_bounds__(CB) void copy_long(const R *rowptr, const uint32_t 
         if (threadIdx.x == 0 && pos != oe)
             atomicOr(bad, 8u);
     }
+}
+
+// Both row copies in one launch: the first nl workgroups take the long rows
+// (dispatched first), the others the short ones, so the two run side by side.
+template <typename R, typename R2>
+__global__ __launch_bounds__(CB) void copy_rows(uint32_t nl, const R *rowptr, const uint32_t *col,
+                                                const LiveWord *lw, const uint32_t *wpre,
+                                                const uint32_t *loc, const uint32_t *longs,
+                                                uint32_t Vn, uint32_t nw, const R2 *orow,
+                                                uint32_t *ocol, uint64_t cap, uint32_t *bad)
+{
+    if (blockIdx.x < nl)
+        copy_long<R, R2>(blockIdx.x, nl, rowptr, col, lw, wpre, loc, longs, Vn, nw, orow, ocol,
+                         cap, bad);
+    else
+        copy_short<R, R2>(blockIdx.x - nl, gridDim.x - nl, rowptr, col, lw, wpre, loc, Vn, nw,
+                          orow, ocol, cap, bad);
 }
 
 __global__ void scatter_kernel(const uint32_t *sub, const uint32_t *gmap, uint32_t n,
@@ -391,12 +406,10 @@ static void launch_copy(const CompactIn &in, const CompactOut &out, hipStream_t 
     uint32_t gs = std::min<uint32_t>(16384, (out.Vn + SR * (CB / KC_WAVE) - 1) /
                                                 (SR * (CB / KC_WAVE)) + 1);
     const uint32_t nw = (in.V + 63) / 64;
-    hipLaunchKernelGGL((copy_short<R, R2>), dim3(gs), dim3(CB), 0, st, (const R *)in.rowptr,
-                       in.col, (const LiveWord *)words_of(out), out.wpre, out.loc, out.Vn, nw, (const R2 *)out.rowptr,
-                       out.col, out.col_cap, out.bad);
     uint32_t gl = std::min<uint32_t>(2048, out.Vn ? out.Vn : 1);
-    hipLaunchKernelGGL((copy_long<R, R2>), dim3(gl), dim3(CB), 0, st, (const R *)in.rowptr, in.col,
-                       (const LiveWord *)words_of(out), out.wpre, out.loc, (const uint32_t *)longs_of(out), out.Vn, nw,
+    hipLaunchKernelGGL((copy_rows<R, R2>), dim3(gl + gs), dim3(CB), 0, st, gl,
+                       (const R *)in.rowptr, in.col, (const LiveWord *)words_of(out), out.wpre,
+                       out.loc, (const uint32_t *)longs_of(out), out.Vn, nw,
                        (const R2 *)out.rowptr, out.col, out.col_cap, out.bad);
 }
 
