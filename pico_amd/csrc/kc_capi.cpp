@@ -852,8 +852,8 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
     p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
     p.hb_ds = use_ds && g->nnz + g->V < 0xffffffffull ? (const uint32_t *)g->hb_ds.p : nullptr;
-    if (use_ds) { // hub list for the rebuild rounds (at most V entries)
-        if ((s = ensure(g, g->hubs, V4, "hub list")))
+    if (use_ds) { // hub lists: all hubs, then those above 4,096 neighbours (V entries each)
+        if ((s = ensure(g, g->hubs, 2 * V4, "hub list")))
             return s;
         p.hubs = (uint32_t *)g->hubs.p;
         p.hub_cap = g->V;

@@ -134,7 +134,7 @@ struct HindexLaunch {
     const uint32_t *col_ds; // degree-ordered adjacency, or null
     const uint32_t *nd_ds;
     const uint32_t *hb_ds;  // histogram base of each col_ds neighbour, or null
-    uint32_t *hubs;         // HistoCore hub list (hub_cap words), or null
+    uint32_t *hubs;         // HistoCore hub lists (2 x hub_cap words), or null
     uint32_t hub_cap;
     GridSync *gs;
     uint64_t *stats;

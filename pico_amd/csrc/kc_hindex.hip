@@ -58,7 +58,8 @@ template <typename R> struct HArgs {
     const uint32_t *col_ds; // degree-ordered adjacency (kc_prep.hip) or null
     const uint32_t *nd_ds;  // neighbour degrees beside col_ds
     const uint32_t *hb_ds;  // neighbour histogram bases beside col_ds (nnz+V < 2^32), or null
-    uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null
+    uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null;
+                            // those of more than KC_HBIG also from hubs + hub_cap on
     uint32_t hub_cap;
     GridSync *gs;
     uint64_t *stats;
@@ -478,6 +479,18 @@ __device__ uint32_t block_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u,
     return block_hist_hyb<R, BLOCK>(L, a, u, cu, a.histo + hbase(a.rowptr, u), n_atom);
 }
 
+// One item index for the whole workgroup from counter ctr (n: item count;
+// none when n is 0).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t giant_grab(HLds<BLOCK> &L, uint32_t *ctr, uint32_t n)
+{
+    __syncthreads(); // the previous index has been read by every thread
+    if (threadIdx.x == 0)
+        L.grab = n ? atomicAdd(ctr, 1u) : 0xffffffffu;
+    __syncthreads();
+    return L.grab;
+}
+
 // ------------------------------------------------------------ grab loop
 // Runs f(item) for items [0, n) of a list (or the vertex range when lst is
 // null), one item per wave; items with more than KC_HBIG neighbours are
@@ -597,6 +610,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
     uint32_t n_big = V; // HistoCore: vertices of more than KC_TINY neighbours
     uint32_t n_hubs = 0;    // HistoCore: listed hubs and their adjacency entries
     uint64_t hub_edges = 0;
+    uint32_t n_giants = 0;  // HistoCore: listed vertices of more than KC_HBIG neighbours
     if (M == HM_HISTO) {
         // Most vertices of a power-law graph have a handful of neighbours: a
         // thread builds such a histogram in registers and stores its d+1
@@ -631,6 +645,24 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                         else
                             sync_fail(gs, 2);
                     }
+                    // the workgroup-sized ones again, counted in flag: their
+                    // first histogram and first count start each phase
+                    bool giant = hub && d > KC_HBIG;
+                    uint64_t gm = ballot(giant);
+                    if (gm) {
+                        uint32_t gl = __ffsll((unsigned long long)gm) - 1;
+                        uint32_t gb = 0;
+                        if (lane == gl)
+                            gb = atomicAdd(&slot_of(gs, phase)[0].flag, (uint32_t)__popcll(gm));
+                        gb = __shfl(gb, gl, KC_WAVE);
+                        if (giant) {
+                            uint32_t p = gb + lane_rank(gm);
+                            if (p < a.hub_cap)
+                                a.hubs[a.hub_cap + p] = v;
+                            else
+                                sync_fail(gs, 2);
+                        }
+                    }
                 }
                 if (valid && !big) {
                     uint64_t s0 = a.rowptr[v], b = s0 + v;
@@ -664,10 +696,27 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
         n_big = ld_relaxed(&slot_of(gs, phase)[0].qtail);
         n_hubs = (uint32_t)ld_relaxed64(&slot_of(gs, phase)[0].sum1);
         hub_edges = ld_relaxed64(&slot_of(gs, phase)[0].sum2);
+        n_giants = ld_relaxed(&slot_of(gs, phase)[0].flag);
         phase = gen;
+        // the largest rows first, a workgroup each, so that none of them is
+        // left for the end of the phase
+        for (;;) {
+            uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
+            if (gi >= n_giants)
+                break;
+            uint32_t v = a.hubs[a.hub_cap + gi];
+            uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+            uint64_t b = hbase(a.rowptr, v);
+            block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+            if (threadIdx.x == 0)
+                a.histo[b] = d;
+            __syncthreads();
+        }
         for_items<BLOCK, R>(
             L, a, phase, n_big, a.list[3],
             [&](uint32_t v, uint32_t d) {
+                if (a.hubs && d > KC_HBIG)
+                    return; // built above
                 uint64_t b = hbase(a.rowptr, v);
                 if (d == 0) { // the reference zeroes the whole array first (:286)
                     if (lane == 0)
@@ -685,6 +734,8 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 }
             },
             [&](uint32_t v) {
+                if (a.hubs)
+                    return; // every item of more than KC_HBIG neighbours was built above
                 uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
                 uint64_t b = hbase(a.rowptr, v);
                 block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
@@ -722,6 +773,8 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             // nodes reserved with a single atomic per grab.
             uint32_t n_st = 0, st_nch = 0;
             auto one = [&](uint32_t v, uint32_t d) {
+                if (round == 1 && a.hubs && d > KC_HBIG)
+                    return; // counted first, below
                 uint32_t old = a.core[v];
                 if (round == 1 && old <= 1)
                     return;
@@ -769,6 +822,54 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 n_st = 0;
                 st_nch = 0;
             };
+            // a changed vertex of more than KC_HBIG neighbours, by the whole
+            // workgroup
+            auto big_count = [&](uint32_t v) {
+                uint32_t old = a.core[v];
+                if (round == 1 && old <= 1)
+                    return;
+                uint64_t b = hbase(a.rowptr, v);
+                uint32_t sum;
+                uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
+                if (k != old) {
+                    uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+                    if (wv == 0) {
+                        uint64_t r = a.nd_ds
+                                         ? wave_count_gt(a.nd_ds + (b - v), d, k)
+                                         : d;
+                        if (lane == 0)
+                            L.bq = (uint32_t)((r + KC_HCH - 1) / KC_HCH);
+                    }
+                    __syncthreads();
+                    uint32_t nch = L.bq;
+                    if (threadIdx.x == 0) {
+                        a.histo[b] = old;
+                        a.histo[b + k] = sum;
+                        a.core[v] = k;
+                        chg_deg += d;
+                        atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0,
+                                  1ull);
+                        L.bk = atomicAdd(&slot_of(gs, phase)[0].qtail, nch);
+                    }
+                    __syncthreads();
+                    uint32_t base = L.bk;
+                    if (base + nch <= a.chunk_cap) {
+                        for (uint32_t c = threadIdx.x; c < nch; c += BLOCK)
+                            a.chunks[base + c] = (uint64_t)v | ((uint64_t)c << 32);
+                    } else if (threadIdx.x == 0) {
+                        sync_fail(gs, 2);
+                    }
+                }
+                __syncthreads();
+            };
+            if (round == 1) { // the largest rows first, a workgroup each
+                for (;;) {
+                    uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
+                    if (gi >= n_giants)
+                        break;
+                    big_count(a.hubs[a.hub_cap + gi]);
+                }
+            }
             if (round == 1) {
                 // the first count of the vertices of at most KC_TINY
                 // neighbours, a thread each: bins from the old core (its
@@ -840,42 +941,9 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             }
             for_items_g<BLOCK, R>(L, a, phase, n_in, in_list, one,
                                 [&](uint32_t v) {
-                                    uint32_t old = a.core[v];
-                                    if (round == 1 && old <= 1)
-                                        return;
-                                    uint64_t b = hbase(a.rowptr, v);
-                                    uint32_t sum;
-                                    uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
-                                    if (k != old) {
-                                        uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
-                                        if (wv == 0) {
-                                            uint64_t r = a.nd_ds
-                                                             ? wave_count_gt(a.nd_ds + (b - v), d, k)
-                                                             : d;
-                                            if (lane == 0)
-                                                L.bq = (uint32_t)((r + KC_HCH - 1) / KC_HCH);
-                                        }
-                                        __syncthreads();
-                                        uint32_t nch = L.bq;
-                                        if (threadIdx.x == 0) {
-                                            a.histo[b] = old;
-                                            a.histo[b + k] = sum;
-                                            a.core[v] = k;
-                                            chg_deg += d;
-                                            atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0,
-                                                      1ull);
-                                            L.bk = atomicAdd(&slot_of(gs, phase)[0].qtail, nch);
-                                        }
-                                        __syncthreads();
-                                        uint32_t base = L.bk;
-                                        if (base + nch <= a.chunk_cap) {
-                                            for (uint32_t c = threadIdx.x; c < nch; c += BLOCK)
-                                                a.chunks[base + c] = (uint64_t)v | ((uint64_t)c << 32);
-                                        } else if (threadIdx.x == 0) {
-                                            sync_fail(gs, 2);
-                                        }
-                                    }
-                                    __syncthreads();
+                                    if (round == 1 && a.hubs)
+                                        return; // counted first, above
+                                    big_count(v);
                                 },
                                 flush);
         } else {
