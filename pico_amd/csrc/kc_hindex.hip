@@ -124,6 +124,23 @@ __device__ __forceinline__ bool hagg_add(HLds<BLOCK> &L, uint32_t word, int32_t 
     return false;
 }
 
+// Items per grab for a list of n: KC_HGRAB when the list is long, fewer when
+// it is short, so that each wave gets about `turns` grabs and a short list is
+// spread over many waves instead of a wave per KC_HGRAB items (an item is
+// several dependent memory round trips).  Measured on R-MAT s22: 2 grabs per
+// wave suit HistoCore, 4 CntCore, 8 NbrCore (whose lists are long in every
+// round).
+__device__ __forceinline__ uint32_t grab_turns(uint32_t method)
+{
+    return method == HM_HISTO ? 2u : method == HM_CNT ? 4u : 8u;
+}
+__device__ __forceinline__ uint32_t grab_size(uint32_t n, uint32_t turns)
+{
+    const uint64_t per = (uint64_t)gridDim.x * (blockDim.x / KC_WAVE) * turns;
+    uint64_t g = (n + per - 1) / per;
+    return g < 1 ? 1u : g > KC_HGRAB ? KC_HGRAB : (uint32_t)g;
+}
+
 // Wave-aggregated append to list `lst` whose counter is this phase's slot.
 __device__ __forceinline__ void hpush(GridSync *gs, uint32_t phase, uint32_t *lst, uint32_t cap,
                                       bool want, uint32_t v)
@@ -504,14 +521,15 @@ __device__ void for_items_g(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, u
     if (threadIdx.x == 0)
         L.nbig = 0;
     __syncthreads();
+    const uint32_t gsz = grab_size(n, grab_turns(a.method));
     for (;;) {
         uint32_t base = 0;
         if (lane == 0)
-            base = atomicAdd(&slot_of(gs, phase)[0].qhead, KC_HGRAB);
+            base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
         base = __shfl(base, 0, KC_WAVE);
         if (base >= n)
             break;
-        uint32_t end = base + KC_HGRAB < n ? base + KC_HGRAB : n;
+        uint32_t end = base + gsz < n ? base + gsz : n;
         for (uint32_t i = base; i < end; i++) {
             uint32_t v = lst ? lst[i] : i;
             uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
@@ -1112,14 +1130,15 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 return flag;
             };
             const uint32_t *adj = a.col_ds ? a.col_ds : a.col;
+            const uint32_t gsz = grab_size(n_chunks, grab_turns(M));
             for (;;) {
                 uint32_t base = 0;
                 if (lane == 0)
-                    base = atomicAdd(&slot_of(gs, phase)[0].qhead, KC_HGRAB);
+                    base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
                 base = __shfl(base, 0, KC_WAVE);
                 if (base >= n_chunks)
                     break;
-                const uint32_t cnt = base + KC_HGRAB < n_chunks ? KC_HGRAB : n_chunks - base;
+                const uint32_t cnt = base + gsz < n_chunks ? gsz : n_chunks - base;
                 // the grab's virtual nodes, one per lane (lanes past cnt hold
                 // an empty range): edge range and the vertex's new / old value
                 uint64_t m_s = 0, m_e = 0;
