@@ -141,6 +141,18 @@ __device__ __forceinline__ uint32_t grab_size(uint32_t n, uint32_t turns)
     return g < 1 ? 1u : g > KC_HGRAB ? KC_HGRAB : (uint32_t)g;
 }
 
+// A short list (at most KC_STATIC items per wave) is dealt out without the
+// shared counter: wave gw takes items gw, gw + nwv, ... with consecutive
+// items on different workgroups.  Thousands of waves each taking a turn on
+// one counter serialise into tens of microseconds, most of a short round.
+constexpr uint32_t KC_STATIC = 4;
+__device__ __forceinline__ bool static_items(uint32_t n, uint32_t &gw, uint32_t &nwv)
+{
+    nwv = gridDim.x * (blockDim.x / KC_WAVE);
+    gw = (threadIdx.x / KC_WAVE) * gridDim.x + blockIdx.x;
+    return (uint64_t)n <= (uint64_t)nwv * KC_STATIC;
+}
+
 // Wave-aggregated append to list `lst` whose counter is this phase's slot.
 __device__ __forceinline__ void hpush(GridSync *gs, uint32_t phase, uint32_t *lst, uint32_t cap,
                                       bool want, uint32_t v)
@@ -521,32 +533,41 @@ __device__ void for_items_g(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t phase, u
     if (threadIdx.x == 0)
         L.nbig = 0;
     __syncthreads();
-    const uint32_t gsz = grab_size(n, grab_turns(a.method));
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0)
-            base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
-        base = __shfl(base, 0, KC_WAVE);
-        if (base >= n)
-            break;
-        uint32_t end = base + gsz < n ? base + gsz : n;
-        for (uint32_t i = base; i < end; i++) {
-            uint32_t v = lst ? lst[i] : i;
-            uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
-            if (d > KC_HBIG) {
-                uint32_t slot = 0;
+    auto item = [&](uint32_t i) {
+        uint32_t v = lst ? lst[i] : i;
+        uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+        if (d > KC_HBIG) {
+            uint32_t slot = 0;
+            if (lane == 0)
+                slot = atomicAdd(&L.nbig, 1u);
+            slot = __shfl(slot, 0, KC_WAVE);
+            if (slot < KC_BIGQ) {
                 if (lane == 0)
-                    slot = atomicAdd(&L.nbig, 1u);
-                slot = __shfl(slot, 0, KC_WAVE);
-                if (slot < KC_BIGQ) {
-                    if (lane == 0)
-                        L.bigq[slot] = v;
-                    continue;
-                }
+                    L.bigq[slot] = v;
+                return;
             }
-            f(v, d);
         }
-        fe(); // end of this wave's grab
+        f(v, d);
+    };
+    uint32_t gw, nwv;
+    if (static_items(n, gw, nwv)) {
+        for (uint32_t i = gw; i < n; i += nwv)
+            item(i);
+        fe();
+    } else {
+        const uint32_t gsz = grab_size(n, grab_turns(a.method));
+        for (;;) {
+            uint32_t base = 0;
+            if (lane == 0)
+                base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
+            base = __shfl(base, 0, KC_WAVE);
+            if (base >= n)
+                break;
+            uint32_t end = base + gsz < n ? base + gsz : n;
+            for (uint32_t i = base; i < end; i++)
+                item(i);
+            fe(); // end of this wave's grab
+        }
     }
     __syncthreads();
     uint32_t nbig = L.nbig < KC_BIGQ ? L.nbig : KC_BIGQ;
@@ -1131,20 +1152,33 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             };
             const uint32_t *adj = a.col_ds ? a.col_ds : a.col;
             const uint32_t gsz = grab_size(n_chunks, grab_turns(M));
+            uint32_t gw, nwv;
+            const bool stat = static_items(n_chunks, gw, nwv);
+            bool stat_done = false;
             for (;;) {
-                uint32_t base = 0;
-                if (lane == 0)
-                    base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
-                base = __shfl(base, 0, KC_WAVE);
-                if (base >= n_chunks)
-                    break;
-                const uint32_t cnt = base + gsz < n_chunks ? gsz : n_chunks - base;
-                // the grab's virtual nodes, one per lane (lanes past cnt hold
+                uint32_t cnt, idx; // virtual nodes of this pass; this lane's index
+                if (stat) { // a short list: nodes gw, gw + nwv, ... (at most KC_STATIC)
+                    if (stat_done || gw >= n_chunks)
+                        break;
+                    stat_done = true;
+                    cnt = (n_chunks - gw + nwv - 1) / nwv;
+                    idx = gw + lane * nwv;
+                } else {
+                    uint32_t base = 0;
+                    if (lane == 0)
+                        base = atomicAdd(&slot_of(gs, phase)[0].qhead, gsz);
+                    base = __shfl(base, 0, KC_WAVE);
+                    if (base >= n_chunks)
+                        break;
+                    cnt = base + gsz < n_chunks ? gsz : n_chunks - base;
+                    idx = base + lane;
+                }
+                // the pass's virtual nodes, one per lane (lanes past cnt hold
                 // an empty range): edge range and the vertex's new / old value
                 uint64_t m_s = 0, m_e = 0;
                 uint32_t m_nc = 0, m_oc = 0;
                 if (lane < cnt) {
-                    uint64_t item = a.chunks[base + lane];
+                    uint64_t item = a.chunks[idx];
                     uint32_t v = (uint32_t)item;
                     uint64_t rs = a.rowptr[v], re = a.rowptr[v + 1];
                     m_s = rs + (uint64_t)(uint32_t)(item >> 32) * KC_HCH;
