@@ -548,11 +548,10 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     uint32_t per_cu = std::min<uint32_t>((uint32_t)occ, block == 1024 ? 1u : 2u);
     uint32_t maxgrid = per_cu * c->num_cus;
     // Default widths measured on R-MAT s22: the level-synchronous GPP and the
-    // PeelOne sweeps are latency-bound and run best on 1/2 and 3/8 of the
-    // CUs; the bucket peel's first stage uses them all.
-    uint32_t defgrid = method == KC_GPP      ? std::max(1u, maxgrid / 2)
-                       : method == KC_PEELONE ? std::max(1u, maxgrid * 3 / 8)
-                                              : maxgrid;
+    // PeelOne sweeps are latency-bound and run best on half of the CUs; the
+    // bucket peel's first stage uses them all.
+    uint32_t defgrid = method == KC_GPP || method == KC_PEELONE ? std::max(1u, maxgrid / 2)
+                                                                : maxgrid;
     uint32_t grid = o.grid ? std::min(o.grid, maxgrid) : defgrid;
     uint32_t hub = o.hub_split ? o.hub_split : 1024;
     if (hub < 64 || hub > (1u << 24))

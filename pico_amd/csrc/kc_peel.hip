@@ -273,12 +273,12 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
             L.agg_cnt[s] = 0;
             uint32_t old = atomicSub(&a.deg[u], c);
             uint32_t eff = old > level ? (old - level < c ? old - level : c) : 0u;
-            na += 1 + (c > eff) + (old > level && old - eff == level && a.mode == MODE_BUCKET);
+            na += 1 + (c > eff) + (old > level && old - eff == level);
             if (c > eff)
                 atomicAdd(&a.deg[u], c - eff);
             if (old - eff > level)
                 atomicMin(&L.minv, old - eff); // degree left above the level
-            if (old > level && old - eff == level && a.mode == MODE_BUCKET) {
+            if (old > level && old - eff == level) {
                 uint64_t bit = 1ull << (u & 63);
                 uint64_t pv = atomicOr((unsigned long long *)&a.done[u >> 6],
                                        (unsigned long long)bit);
@@ -349,18 +349,20 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
             if (!hit[k])
                 continue;
             na += 1 + (a.mode != MODE_GPP && old[k] <= level) +
-                  (a.mode == MODE_BUCKET && old[k] == level + 1);
+                  (a.mode != MODE_GPP && old[k] == level + 1);
             if (old[k] > level + 1 && old[k] - 1 < nmin)
                 nmin = old[k] - 1;
             if (a.mode == MODE_GPP) {
                 claim[k] = old[k] == level + 1;
             } else if (old[k] == level + 1) {
-                if (a.mode != MODE_PEELONE) { // PeelOne: a later sweep fires it
-                    uint64_t bit = 1ull << (u[k] & 63);
-                    uint64_t pv = atomicOr((unsigned long long *)&a.done[u[k] >> 6],
-                                           (unsigned long long)bit);
-                    claim[k] = (pv & bit) == 0;
-                }
+                // bucket peels and PeelOne alike: a vertex lowered onto the
+                // level fires within this sweep (peelone.cu's parts fire in
+                // the launch that finds dist == level, whichever thread
+                // lowered it)
+                uint64_t bit = 1ull << (u[k] & 63);
+                uint64_t pv = atomicOr((unsigned long long *)&a.done[u[k] >> 6],
+                                       (unsigned long long)bit);
+                claim[k] = (pv & bit) == 0;
             } else if (old[k] <= level) {
                 atomicAdd(&a.deg[u[k]], 1u); // clamp-at-level restore
             }
@@ -1163,8 +1165,19 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         if (total >= a.V)
             break;
         if (a.mode == MODE_PEELONE) {
-            if (lvl_claims == 0)
-                level++;
+            // peelone.cu's host loop (:154-186) sweeps a level again after a
+            // sweep that fired and moves on after one that did not.  Every
+            // vertex lowered onto the level fired in this sweep, so that
+            // repeat finds nothing, and neither does the sweep of each empty
+            // level below the next live degree: count those sweeps, run none.
+            if (lvl_claims)
+                iters++;
+            if (mn == 0xffffffffu || mn <= level) {
+                sync_fail(gs, 3);
+                break;
+            }
+            iters += (uint64_t)(mn - level - 1);
+            level = mn;
         } else {
             // mn is the smallest degree of any vertex still live: the scan's
             // minimum over untouched vertices and every decrement's result
