@@ -626,7 +626,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     // remain (kc_compact.hip); stage 0 is the graph itself.
     const uint32_t cdiv = o.compact_div ? o.compact_div : 8u;
     auto tail_of = [&](uint32_t Vs) -> uint32_t {
-        if (mode != 0 || cdiv == 0xffffffffu || cdiv < 2)
+        if (mode == 2 || cdiv == 0xffffffffu || cdiv < 2) // not GPP
             return 0u;
         uint32_t t = Vs / cdiv;
         return t >= 4096u ? t : 0u;
@@ -639,7 +639,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     if (e != hipSuccess)
         return hip_fail(e, "peel launch");
     uint64_t sv[kc::KC_ST_COUNT];
-    uint64_t acc_levels = 0, acc_phases = 0, acc_edges = 0, acc_atomics = 0;
+    uint64_t acc_levels = 0, acc_phases = 0, acc_edges = 0, acc_atomics = 0, acc_iters = 0;
     bool compacted = false;
     uint32_t nstages = 1;
     uint64_t stage_nnz = g->nnz; // bound on the current stage's adjacency
@@ -664,6 +664,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         acc_phases += sv[kc::KC_ST_PHASES];
         acc_edges += sv[kc::KC_ST_EDGES];
         acc_atomics += sv[kc::KC_ST_ATOMICS];
+        acc_iters += sv[kc::KC_ST_ITERS];
         const int cur = stage & 1, nxt = (stage + 1) & 1;
         if (stage > 0)
             KC_HIP(kc::scatter_cores(p.deg, (const uint32_t *)g->cs_gmap[cur].p, p.V,
@@ -772,7 +773,9 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     if (method == KC_PEEL_DYN || method == KC_PEELONE_DYN)
         st.iterations = st.levels; // peel-dyn.cc:155-173: one trip per level
     else
-        st.iterations = sv[kc::KC_ST_ITERS];
+        // GPP runs in one launch.  PeelOne's stages add up; each launch
+        // counts the sweep that ends the host loop, reached only by the last
+        st.iterations = acc_iters - (nstages - 1);
     return KC_OK;
 }
 

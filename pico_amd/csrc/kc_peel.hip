@@ -992,7 +992,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     }
 
     uint32_t gen = 0;
-    uint32_t level = (a.mode == MODE_GPP) ? 1u : (a.mode == MODE_BUCKET ? a.start_level : 0u);
+    uint32_t level = (a.mode == MODE_GPP) ? 1u : a.start_level;
     uint32_t tail_level = 0xffffffffu; // level at which the kernel stopped early
     uint64_t total = 0; // vertices claimed so far (identical in all WGs)
     uint64_t lvl_claims = 0;
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         // Few vertices left: stop before this level's scan so the caller can
         // compact the live subgraph and continue on it (the state is that of
         // a completed level; every workgroup takes the same decision).
-        if (a.mode == MODE_BUCKET && kind == PH_SCAN && a.tail_live && total < a.V &&
+        if (a.mode != MODE_GPP && kind == PH_SCAN && a.tail_live && total < a.V &&
             (uint64_t)a.V - total <= a.tail_live) {
             tail_level = level;
             break;
@@ -1265,9 +1265,9 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.out = p.out;
     a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
-    a.tail_live = p.mode == MODE_BUCKET ? p.tail_live : 0u;
+    a.tail_live = p.mode != MODE_GPP ? p.tail_live : 0u;
     a.abort_flag = p.abort_flag;
-    a.start_level = p.mode == MODE_BUCKET ? p.start_level : 0u;
+    a.start_level = p.mode != MODE_GPP ? p.start_level : 0u;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
