@@ -196,3 +196,28 @@ def test_compacted_stages(ctx, method, div, gen):
     assert st0["stages"] == 1
     if div <= 4:  # V/div >= 4,096 live vertices: at least one compaction
         assert st["stages"] >= 2
+
+
+@pytest.mark.parametrize("gen", ["rmat15", "chunglu15"])
+def test_peelone_compacted_stages(ctx, gen):
+    # PeelOne continues on compacted live subgraphs too: same cores, kmax and
+    # "Number of iterations" as in one launch (each stopped launch's count of
+    # the loop-ending sweep is taken back), at least one compaction
+    if gen == "rmat15":
+        hg = pa.HostGraph.rmat(15, 16, seed=9)
+        dg = ctx.upload(hg)
+    else:
+        hg = pa.HostGraph.chunglu(1 << 15, beta=2.1, mean_deg=20.0, cap=4000.0, seed=4)
+        dg = ctx.upload(hg, rowptr_bytes=8)
+    try:
+        core, st = dg.decompose("peelone", compact_div=4)
+        ref, st0 = dg.decompose("peelone", compact_div=0xFFFFFFFF)
+    finally:
+        dg.close()
+    want = oracle.bz(hg.rowptr, hg.col)
+    np.testing.assert_array_equal(core, want)
+    np.testing.assert_array_equal(ref, want)
+    assert st["kmax"] == st0["kmax"] == int(want.max())
+    assert st0["stages"] == 1 and st["stages"] >= 2
+    assert st["iterations"] == st0["iterations"]
+    assert st["levels_run"] == st0["levels_run"]
