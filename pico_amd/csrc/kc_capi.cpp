@@ -520,8 +520,12 @@ static std::mutex &coop_mutex(int device)
     return mu[(unsigned)device & 63];
 }
 
-__global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
+// Largest value of a[0..n): one atomic per workgroup (256 threads), not per
+// wave -- thousands of same-address atomics serialise into tens of us.
+__global__ __launch_bounds__(256) void kc_max_kernel(const uint32_t *a, uint32_t n,
+                                                     uint32_t *out)
 {
+    __shared__ uint32_t s_m[256 / 64];
     uint32_t m = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         m = a[i] > m ? a[i] : m;
@@ -529,8 +533,15 @@ __global__ void kc_max_kernel(const uint32_t *a, uint32_t n, uint32_t *out)
         uint32_t y = __shfl_xor(m, o, 64);
         m = y > m ? y : m;
     }
-    if ((threadIdx.x & 63) == 0 && m)
-        atomicMax(out, m);
+    if ((threadIdx.x & 63) == 0)
+        s_m[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / 64; w++)
+            m = s_m[w] > m ? s_m[w] : m;
+        if (m)
+            atomicMax(out, m);
+    }
 }
 
 // Default grid width for a graph of nnz adjacency entries: a smaller graph's
@@ -751,7 +762,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     KC_HIP(hipMemsetAsync(g->kmax.p, 0, 4, c->stream), "memset");
     g->result = mode == 2 ? (const uint32_t *)g->out.p : (const uint32_t *)g->deg.p;
     if (g->V)
-        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(1024, (g->V + 255) / 256)),
+        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(512, (g->V + 255) / 256)),
                            dim3(256), 0, c->stream, g->result, g->V, (uint32_t *)g->kmax.p);
     KC_HIP(hipStreamSynchronize(c->stream), "decomposition");
     float ms = 0;
@@ -898,7 +909,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     KC_HIP(hipMemsetAsync(g->kmax.p, 0, 4, c->stream), "memset");
     g->result = (const uint32_t *)g->deg.p;
     if (g->V)
-        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(1024, (g->V + 255) / 256)),
+        hipLaunchKernelGGL(kc_max_kernel, dim3(std::min<uint32_t>(512, (g->V + 255) / 256)),
                            dim3(256), 0, c->stream, g->result, g->V, (uint32_t *)g->kmax.p);
     KC_HIP(hipStreamSynchronize(c->stream), "decomposition");
     float ms = 0;
