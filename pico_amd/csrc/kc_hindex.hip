@@ -1443,15 +1443,30 @@ out:
         a.stats[KC_ST_CHANGED] = changed_total;
         a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
     }
-    uint64_t ue = wave_sum_u64(upd_edges);
-    if (lane == 0 && ue)
-        atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], (unsigned long long)ue);
-    uint64_t cd = wave_sum_u64(chg_deg);
-    if (lane == 0 && cd)
-        atomicAdd((unsigned long long *)&a.stats[KC_ST_CHG_DEG], (unsigned long long)cd);
-    uint64_t na = wave_sum_u64(n_atom);
-    if (lane == 0 && na)
-        atomicAdd((unsigned long long *)&a.stats[KC_ST_ATOMICS], (unsigned long long)na);
+    // per-workgroup sums first: one global atomic per counter and workgroup
+    // (every path to here is taken by the whole workgroup)
+    __shared__ unsigned long long s_sum[3];
+    if (threadIdx.x < 3)
+        s_sum[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t ue = wave_sum_u64(upd_edges), cd = wave_sum_u64(chg_deg), na = wave_sum_u64(n_atom);
+    if (lane == 0) {
+        if (ue)
+            atomicAdd(&s_sum[0], (unsigned long long)ue);
+        if (cd)
+            atomicAdd(&s_sum[1], (unsigned long long)cd);
+        if (na)
+            atomicAdd(&s_sum[2], (unsigned long long)na);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_sum[0])
+            atomicAdd((unsigned long long *)&a.stats[KC_ST_EDGES], s_sum[0]);
+        if (s_sum[1])
+            atomicAdd((unsigned long long *)&a.stats[KC_ST_CHG_DEG], s_sum[1]);
+        if (s_sum[2])
+            atomicAdd((unsigned long long *)&a.stats[KC_ST_ATOMICS], s_sum[2]);
+    }
 }
 
 template <typename R>
