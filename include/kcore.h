@@ -72,7 +72,9 @@ typedef enum kc_method {
 /* Launch options; zero fields select defaults (or the autotuner's pick). */
 typedef struct kc_opts {
     uint32_t block;      /* threads per workgroup: 256, 512 or 1024 */
-    uint32_t grid;       /* persistent workgroups (<= occupancy * CUs) */
+    uint32_t grid;       /* persistent workgroups (<= occupancy * CUs); 0: the method's
+                            default, narrowed for graphs of fewer than ~30M adjacency
+                            entries to about 48 * sqrt(nnz / 1e6), at least 32 */
     uint32_t hub_split;  /* edges per virtual node of a hub vertex */
     uint32_t queue_cap;  /* shared work-list capacity in items */
     uint32_t gpp_compat; /* KC_GPP: report isolated vertices as 1 (reference quirk) */
@@ -83,12 +85,12 @@ typedef struct kc_opts {
     uint32_t agg_div;    /* bucket peels: read each target's degree before decrementing and
                             sum hub decrements in LDS once at most V/agg_div vertices are
                             live (0: 64; 0xffffffff: from the start) */
-    uint32_t compact_div; /* bucket peels: once at most V/compact_div vertices are live (and
+    uint32_t compact_div; /* bucket peels and PeelOne: once at most V/compact_div vertices are live (and
                              at least 4,096), continue on a compacted copy of the live
                              subgraph, again at each further 1/compact_div; 0: default (8),
                              0xffffffff: never */
-    uint32_t stage_grid; /* bucket peels: persistent workgroups for the compacted stages
-                            (<= occupancy * CUs); 0: the grid of the first stage */
+    uint32_t stage_grid; /* bucket peels and PeelOne: persistent workgroups for the compacted
+                            stages (<= occupancy * CUs); 0: 3/8 of the largest resident grid */
     uint32_t reserved[1];
 } kc_opts;
 
