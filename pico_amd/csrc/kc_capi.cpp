@@ -613,7 +613,14 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.list_cap = cap;
     p.mode = mode;
     p.out = (uint32_t *)g->out.p;
-    p.live_thresh = g->V / 4; // switch to the live-vertex list below a quarter of V
+    if (mode == 2) { // GPP keeps its cores in out: its live-vertex list goes elsewhere
+        if ((s = ensure(g, g->hl0, (size_t)(g->V ? g->V : 1) * 4, "live list")))
+            return s;
+        p.live = (uint32_t *)g->hl0.p;
+    }
+    // switch to the live-vertex list below a quarter of V (GPP: half, as its
+    // every level starts with a full scan)
+    p.live_thresh = mode == 2 ? g->V / 2 : g->V / 4;
     // the dense core: pre-check + aggregation once at most V/agg_div remain
     uint32_t agg_div = o.agg_div ? o.agg_div : 64;
     p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : g->V / agg_div;

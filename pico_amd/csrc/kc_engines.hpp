@@ -52,7 +52,8 @@ struct PeelLaunch {
     uint32_t list_cap;
     uint32_t mode; // 0 bucket, 1 PeelOne sweeps, 2 GPP
     uint32_t *out; // GPP final core; bucket: the live-vertex list (V entries)
-    uint32_t live_thresh; // bucket: scan a live-vertex list once <= this many remain
+    uint32_t *live; // GPP: its live-vertex list (V entries), or null
+    uint32_t live_thresh; // bucket, GPP: scan a live-vertex list once <= this many remain
     uint32_t agg_live;    // pre-check degrees and aggregate hub decrements once <= this many remain
     uint32_t tail_live;   // bucket: stop at a level start once <= this many remain (0: never)
     uint32_t start_level; // bucket: first level to visit

@@ -38,7 +38,8 @@ template <typename R> struct PeelArgs {
     uint64_t *list1;
     uint32_t list_cap;
     uint32_t mode;
-    uint32_t *out; // GPP: final core; bucket: live-vertex list
+    uint32_t *out;  // GPP: final core
+    uint32_t *live; // live-vertex list (bucket: out; GPP: a buffer of its own), or null
     uint32_t live_thresh;
     uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
     uint32_t tail_live;   // bucket: stop at a level start once <= this many are live
@@ -636,11 +637,11 @@ __device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
         while (free && pos < end) {
             uint32_t b = __ffsll((unsigned long long)free) - 1;
             free &= free - 1;
-            a.out[pos++] = c * 64 + b;
+            a.live[pos++] = c * 64 + b;
         }
     }
     while (pos < end)
-        a.out[pos++] = KC_DEAD;
+        a.live[pos++] = KC_DEAD;
     __syncthreads();
 }
 
@@ -651,7 +652,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
     constexpr uint32_t U = 4;
     const uint32_t base = L.lbase, n = L.lcnt;
     const uint32_t lane = lane_id();
-    uint32_t *lst = a.out + base;
+    uint32_t *lst = a.live + base;
     uint32_t lmin = 0xffffffffu, ncand = 0;
     bool live_seen = false;
     if (threadIdx.x == 0)
@@ -692,7 +693,16 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                                        (unsigned long long)bit);
                 claim = (pv & bit) == 0;
             }
-            push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v[k]);
+            if (a.mode == MODE_GPP) { // retire and list for the scatter, as scan_chunk
+                if (claim)
+                    a.out[v[k]] = level;
+                uint64_t rm = ballot(claim);
+                if (lane == 0 && rm)
+                    atomicAdd(&L.claims, (uint32_t)__popcll(rm));
+                push_shared(a, phase, claim, (uint64_t)v[k]);
+            } else {
+                push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v[k]);
+            }
             uint64_t km = ballot(keep);
             if (km) {
                 uint32_t leader = __ffsll((unsigned long long)km) - 1;
@@ -1053,7 +1063,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             atomicMax((unsigned long long *)&tr[14], (unsigned long long)t);
         }
         if (kind == PH_SCAN) {
-            if (a.mode == MODE_BUCKET && !L.lmode && a.live_thresh &&
+            if (a.mode != MODE_PEELONE && !L.lmode && a.live_thresh &&
                 (uint64_t)a.V - total <= a.live_thresh)
                 build_live_list<BLOCK, LQ, R, CNT>(L, a);
             if (L.lmode)
@@ -1263,7 +1273,8 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.list_cap = p.list_cap;
     a.mode = p.mode;
     a.out = p.out;
-    a.live_thresh = p.mode == MODE_BUCKET && p.out ? p.live_thresh : 0u;
+    a.live = p.mode == MODE_GPP ? p.live : p.out;
+    a.live_thresh = p.mode != MODE_PEELONE && a.live ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
     a.tail_live = p.mode != MODE_GPP ? p.tail_live : 0u;
     a.abort_flag = p.abort_flag;
