@@ -175,6 +175,37 @@ __device__ __forceinline__ void hpush(GridSync *gs, uint32_t phase, uint32_t *ls
     }
 }
 
+// hpush of N lanes' worth at once: one counter atomic for all N ballots.
+template <int N>
+__device__ __forceinline__ void hpush_n(GridSync *gs, uint32_t phase, uint32_t *lst, uint32_t cap,
+                                        const bool (&want)[N], const uint32_t (&v)[N])
+{
+    uint64_t m[N];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        m[k] = ballot(want[k]);
+        tot += (uint32_t)__popcll(m[k]);
+    }
+    if (!tot)
+        return;
+    uint32_t base = 0;
+    if (lane_id() == 0)
+        base = atomicAdd(&slot_of(gs, phase)[0].qtail, tot);
+    base = __shfl(base, 0, KC_WAVE);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (want[k]) {
+            uint32_t p = base + lane_rank(m[k]);
+            if (p < cap)
+                lst[p] = v[k];
+            else
+                sync_fail(gs, 2);
+        }
+        base += (uint32_t)__popcll(m[k]);
+    }
+}
+
 // Top-down search over bins[1..top] for the largest k <= top with
 // sum_{j>=k} bins[j] >= k.  Returns k (0 if none) and the suffix sum at k.
 // `global_bins` selects coherent loads for HBM bins.
@@ -1223,9 +1254,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                             f[j] = patch(u[j], bu[j], a.hb_ds && nd[j] > KC_HUB_AGG, cu[j], nc[j],
                                          oc[j]);
                     }
-#pragma unroll
-                    for (uint32_t j = 0; j < KC_WALK_U; j++)
-                        hpush(gs, phase, f_list, cap_list, f[j], u[j]);
+                    hpush_n<KC_WALK_U>(gs, phase, f_list, cap_list, f, u);
                 }
             }
             // Apply the summed hub updates.  A top bin only ever decreases in
@@ -1309,9 +1338,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                                     a.cnt[u[k]] = 0;
                             }
                         }
-#pragma unroll
-                        for (int k = 0; k < 4; k++)
-                            hpush(gs, phase, m_list, cap_list, fresh[k], u[k]);
+                        hpush_n<4>(gs, phase, m_list, cap_list, fresh, u);
                     }
                 },
                 [&](uint32_t v) {
@@ -1339,9 +1366,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                                     a.cnt[u[k]] = 0;
                             }
                         }
-#pragma unroll
-                        for (int k = 0; k < 4; k++)
-                            hpush(gs, phase, m_list, cap_list, fresh[k], u[k]);
+                        hpush_n<4>(gs, phase, m_list, cap_list, fresh, u);
                     }
                     __syncthreads();
                 });
