@@ -800,7 +800,9 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
 static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_stats &st)
 {
     kc_ctx *c = g->ctx;
-    uint32_t block = o.block ? o.block : 512;
+    // HistoCore runs best with 512-thread groups, CntCore / NbrCore with 1,024
+    // (R-MAT s22 and the Chung-Lu graph)
+    uint32_t block = o.block ? o.block : method == KC_HISTO ? 512u : 1024u;
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
     const uint32_t mi = method == KC_HISTO ? 0 : method == KC_CNT ? 1 : 2;
