@@ -71,7 +71,8 @@ typedef enum kc_method {
 
 /* Launch options; zero fields select defaults (or the autotuner's pick). */
 typedef struct kc_opts {
-    uint32_t block;      /* threads per workgroup: 256, 512 or 1024 */
+    uint32_t block;      /* threads per workgroup: 256, 512 or 1024; 0: 512 for HistoCore,
+                            1024 for the other methods */
     uint32_t grid;       /* persistent workgroups (<= occupancy * CUs); 0: the method's
                             default, narrowed for graphs of fewer than ~30M adjacency
                             entries to about 48 * sqrt(nnz / 1e6), at least 32 */
