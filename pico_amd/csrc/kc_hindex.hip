@@ -78,7 +78,7 @@ constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggrega
 // more than KC_PULL_T neighbours) hold, every hub's bins are recounted from its
 // neighbours' values (reads) instead of patched by two atomics per affected
 // edge; updates aimed at the other vertices are patched as usual.
-constexpr uint32_t KC_PULL_T = KC_HUB_AGG;
+constexpr uint32_t KC_PULL_T = 512;
 constexpr uint64_t KC_PULL_NUM = 1, KC_PULL_DEN = 4;
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
