@@ -79,7 +79,7 @@ constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggrega
 // neighbours' values (reads) instead of patched by two atomics per affected
 // edge; updates aimed at the other vertices are patched as usual.
 constexpr uint32_t KC_PULL_T = 512;
-constexpr uint64_t KC_PULL_NUM = 1, KC_PULL_DEN = 4;
+constexpr uint64_t KC_PULL_NUM = 1, KC_PULL_DEN = 3;
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
 template <int BLOCK> struct HLds {
