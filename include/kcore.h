@@ -118,6 +118,8 @@ typedef struct kc_stats {
                          arrays: degrees and claim bitmap (peel, with KC_OPT_COUNT;
                          0 without), histograms and marks (h-index, always); queue
                          counters are not included */
+    uint64_t rebuild_rounds; /* HistoCore: update rounds that recounted every hub's
+                                bins from its neighbours instead of patching them */
 } kc_stats;
 
 KC_API const char *kc_last_error(void);
@@ -350,6 +352,33 @@ KC_API kc_status kc_graph_gen_chunglu(kc_ctx *ctx, uint32_t n, double beta, doub
                                       double cap, uint64_t seed, uint32_t rowptr_bytes,
                                       kc_graph **out);
 KC_API kc_status kc_graph_download(const kc_graph *g, kc_host_graph **out);
+
+/* An edge file straight into a resident graph: the replacement of
+ * Graph::ReadGraph + VirtualGraph (peel/shared/graph.cpp:17-99,
+ * virtual_graph.cpp:9-157; reader KC_READER_COO) and of Graph(string) /
+ * readFile (peel/src/graph.cpp:58-133; KC_READER_DEDUP) followed by the
+ * upload.  The file is memory-mapped and parsed by host threads in chunks of
+ * whole lines (results as one sequential pass, reader quirks included), and
+ * the CSR is built on the device: a stable radix sort of the directed entries
+ * for the COO reader (duplicates kept, rows in file order), a sort + unique
+ * for the deduplicating reader (rows sorted).  The graph is entry for entry
+ * the one kc_hg_read_coo / kc_hg_read_dedup + kc_graph_upload give.
+ * stats may be NULL. */
+typedef struct kc_read_stats {
+    uint32_t V;
+    uint32_t has_zero_id; /* COO reader: some id was 0 (no 1-based shift) */
+    uint32_t multigraph;  /* COO reader: duplicates kept */
+    uint32_t threads;     /* parse threads */
+    uint64_t nnz;         /* adjacency entries of the resident graph */
+    uint64_t edges_read;  /* edge lines kept (self-loops dropped) */
+    uint64_t file_bytes;
+    double parse_ms;  /* map + parse on the host */
+    double build_ms;  /* upload of the pairs + device CSR build */
+    double total_ms;  /* host wall time of the call */
+    uint64_t reserved[4];
+} kc_read_stats;
+KC_API kc_status kc_graph_read(kc_ctx *ctx, const char *path, int reader, uint32_t rowptr_bytes,
+                               kc_graph **out, kc_read_stats *stats);
 KC_API kc_status kc_batch_gen_rmat(kc_ctx *ctx, uint32_t G, uint32_t scale_min,
                                    uint32_t scale_max, double ef_min, double ef_max, double a,
                                    double b, double c, uint64_t seed, kc_batch **out);

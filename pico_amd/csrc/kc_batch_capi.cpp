@@ -213,6 +213,7 @@ kc_status graph_adopt(kc_ctx *ctx, uint32_t V, uint64_t nnz, bool wide, void *ro
         return fail(KC_ENOMEM, "host allocation");
     }
     g->ctx = ctx;
+    ctx_ref(ctx);
     g->V = V;
     g->nnz = nnz;
     g->wide = wide;
@@ -353,6 +354,7 @@ kc_status kc_batch_upload(kc_ctx *ctx, uint32_t G, const uint64_t *vtx_off,
     if (!b)
         return fail(KC_ENOMEM, "host allocation");
     b->ctx = ctx;
+    ctx_ref(ctx);
     b->G = G;
     b->Vtot = vtx_off[G];
     b->nnz = edge_off[G];
@@ -411,6 +413,62 @@ kc_status kc_graph_gen_rmat(kc_ctx *ctx, uint32_t scale, uint64_t m_gen, double 
                     err.c_str());
     }
     return graph_from_build(ctx, (uint32_t)vo[1], rowptr_bytes, bo, out);
+}
+
+kc_status kc_graph_read(kc_ctx *ctx, const char *path, int reader, uint32_t rowptr_bytes,
+                        kc_graph **out, kc_read_stats *stats)
+{
+    if (!ctx || !path || !out)
+        return fail(KC_EINVAL, "null argument");
+    if (reader != KC_READER_COO && reader != KC_READER_DEDUP)
+        return fail(KC_EINVAL, "unknown reader %d", reader);
+    if (rowptr_bytes != 4 && rowptr_bytes != 8)
+        return fail(KC_EINVAL, "rowptr_bytes must be 4 or 8");
+    const double t0 = now_ms();
+    kch::EdgeFile ef;
+    std::string err;
+    int rc = 0;
+    try {
+        rc = reader == KC_READER_COO ? kch::read_coo_pairs(path, ef, err)
+                                     : kch::read_dedup_pairs(path, ef, err);
+    } catch (const std::bad_alloc &) {
+        return fail(KC_ENOMEM, "host allocation for the edges of %s failed", path);
+    }
+    if (rc == -1)
+        return fail(KC_EIO, "%s", err.c_str());
+    if (rc)
+        return fail(KC_EINVAL, "%s: %s", path, err.c_str());
+    const double t1 = now_ms();
+    KC_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    const uint64_t m = ef.src.size();
+    const bool keep = reader == KC_READER_COO;
+    const uint64_t nnz_max = 2 * m;
+    const bool wide = rowptr_bytes == 8 || nnz_max > 0xffffffffull;
+    kc::PairCsr pc;
+    hipError_t e = kc::build_csr_pairs(ef.src.data(), ef.dst.data(), m, ef.V, keep, wide,
+                                       ctx->stream, pc, err);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? KC_ENOMEM : KC_EINVAL, "device CSR build: %s",
+                    err.c_str());
+    const double t2 = now_ms();
+    kc_status s = graph_adopt(ctx, ef.V, pc.nnz, pc.wide, pc.rowptr, pc.col, out);
+    if (s)
+        return s;
+    if (stats) {
+        kc_read_stats r{};
+        r.V = ef.V;
+        r.has_zero_id = ef.has_zero_id;
+        r.multigraph = keep;
+        r.threads = ef.threads;
+        r.nnz = pc.nnz;
+        r.edges_read = m;
+        r.file_bytes = ef.file_bytes;
+        r.parse_ms = t1 - t0;
+        r.build_ms = t2 - t1;
+        r.total_ms = now_ms() - t0;
+        *stats = r;
+    }
+    return KC_OK;
 }
 
 kc_status kc_graph_gen_chunglu(kc_ctx *ctx, uint32_t n, double beta, double mean_deg, double cap,
@@ -630,6 +688,7 @@ kc_status kc_batch_gen_rmat_list(kc_ctx *ctx, uint32_t G, const uint32_t *scale,
     if (!bt)
         return fail(KC_ENOMEM, "host allocation");
     bt->ctx = ctx;
+    ctx_ref(ctx);
     bt->G = G;
     bt->scale.assign(scale, scale + G);
     bt->m_gen.assign(m_gen, m_gen + G);
@@ -813,6 +872,7 @@ kc_status kc_batch_run_plain(kc_batch *b, kc_method method, const kc_opts *opts,
         if (!b->view)
             return fail(KC_ENOMEM, "host allocation");
         b->view->ctx = b->ctx;
+        ctx_ref(b->ctx);
         b->view->owns_csr = false;
         KC_HIP(hipEventCreate(&b->view->ev0), "hipEventCreate");
         KC_HIP(hipEventCreate(&b->view->ev1), "hipEventCreate");
@@ -940,7 +1000,9 @@ void kc_batch_free(kc_batch *b)
         hipEventDestroy(b->ev0);
     if (b->ev1)
         hipEventDestroy(b->ev1);
+    kc_ctx *c = b->ctx;
     delete b;
+    ctx_unref(c);
 }
 
 // ------------------------------------------------------------ calibration

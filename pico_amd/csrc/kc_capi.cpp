@@ -60,6 +60,22 @@ kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what)
     return KC_OK;
 }
 
+void ctx_ref(kc_ctx *c)
+{
+    if (c)
+        c->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+void ctx_unref(kc_ctx *c)
+{
+    if (!c || c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1)
+        return;
+    hipSetDevice(c->device);
+    if (c->stream)
+        hipStreamDestroy(c->stream);
+    delete c;
+}
+
 } // namespace kci
 
 using namespace kci;
@@ -70,6 +86,29 @@ const char *kc_last_error(void) { return g_err.c_str(); }
 const char *kc_version(void) { return "kcore-mi355x 0.1 (gfx950)"; }
 
 // ------------------------------------------------------------ host graphs
+// Run a host reader: status from its return code, and no C++ exception
+// (an edge file whose ids ask for more memory than the host has) leaves
+// through the C ABI.
+static kc_status run_reader(int reader, const char *path, kc_host_graph &g)
+{
+    std::string err;
+    int rc = 0;
+    try {
+        rc = reader == KC_READER_COO ? kch::read_coo(path, g, err) : kch::read_dedup(path, g, err);
+    } catch (const std::bad_alloc &) {
+        g = kc_host_graph{};
+        return fail(KC_ENOMEM, "host allocation for the graph of %s failed", path);
+    } catch (const std::exception &x) {
+        g = kc_host_graph{};
+        return fail(KC_EINVAL, "%s: %s", path, x.what());
+    }
+    if (rc == -1)
+        return fail(KC_EIO, "%s", err.c_str());
+    if (rc)
+        return fail(KC_EINVAL, "%s: %s", path, err.c_str());
+    return KC_OK;
+}
+
 kc_status kc_hg_read_coo(const char *path, kc_host_graph **out)
 {
     if (!path || !out)
@@ -77,10 +116,10 @@ kc_status kc_hg_read_coo(const char *path, kc_host_graph **out)
     auto *g = new (std::nothrow) kc_host_graph;
     if (!g)
         return fail(KC_ENOMEM, "host allocation");
-    std::string err;
-    if (kch::read_coo(path, *g, err)) {
+    kc_status s = run_reader(KC_READER_COO, path, *g);
+    if (s) {
         delete g;
-        return fail(KC_EIO, "%s", err.c_str());
+        return s;
     }
     *out = g;
     return KC_OK;
@@ -142,11 +181,10 @@ kc_status kc_hg_read_cached(const char *path, int reader, const char *cache_dir,
     }
     // miss (absent, stale or damaged entry): parse, then (re)write the entry
     *g = kc_host_graph{};
-    int rc = reader == KC_READER_COO ? kch::read_coo(path, *g, err)
-                                     : kch::read_dedup(path, *g, err);
-    if (rc) {
+    kc_status rs = run_reader(reader, path, *g);
+    if (rs) {
         delete g;
-        return fail(err.rfind("cannot", 0) == 0 ? KC_EIO : KC_EINVAL, "%s", err.c_str());
+        return rs;
     }
     if (kch::save_csr(*g, entry.c_str(), (uint32_t)reader, hash, size, err)) {
         delete g;
@@ -163,10 +201,10 @@ kc_status kc_hg_read_dedup(const char *path, kc_host_graph **out)
     auto *g = new (std::nothrow) kc_host_graph;
     if (!g)
         return fail(KC_ENOMEM, "host allocation");
-    std::string err;
-    if (kch::read_dedup(path, *g, err)) {
+    kc_status s = run_reader(KC_READER_DEDUP, path, *g);
+    if (s) {
         delete g;
-        return fail(err.rfind("cannot", 0) == 0 ? KC_EIO : KC_EINVAL, "%s", err.c_str());
+        return s;
     }
     *out = g;
     return KC_OK;
@@ -357,15 +395,7 @@ kc_status kc_ctx_create(int device, kc_ctx **out)
     return KC_OK;
 }
 
-void kc_ctx_destroy(kc_ctx *ctx)
-{
-    if (!ctx)
-        return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream)
-        hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
+void kc_ctx_destroy(kc_ctx *ctx) { kci::ctx_unref(ctx); }
 
 kc_status kc_ctx_info(const kc_ctx *ctx, uint32_t *num_cus, uint64_t *hbm_bytes)
 {
@@ -405,6 +435,7 @@ kc_status kc_graph_upload(kc_ctx *ctx, uint32_t V, const uint64_t *rowptr, const
     if (!g)
         return fail(KC_ENOMEM, "host allocation");
     g->ctx = ctx;
+    ctx_ref(ctx);
     g->V = V;
     g->nnz = nnz;
     g->wide = wide;
@@ -475,7 +506,9 @@ void kc_graph_free(kc_graph *g)
     if (g->ev1)
         hipEventDestroy(g->ev1);
     dyn_free(g->dyn);
+    kc_ctx *c = g->ctx;
     delete g;
+    ctx_unref(c);
 }
 
 uint64_t kc_graph_device_bytes(const kc_graph *g) { return g ? g->bytes : 0; }
@@ -941,6 +974,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     st.changed = sv[kc::KC_ST_CHANGED];
     st.changed_degree = sv[kc::KC_ST_CHG_DEG];
     st.atomics = sv[kc::KC_ST_ATOMICS];
+    st.rebuild_rounds = sv[kc::KC_ST_REBUILDS];
     if (p.trace)
         g->trace_phases = (uint32_t)std::min<uint64_t>(st.phases, KC_TRACE_CAP);
     st.grid = grid;

@@ -11,6 +11,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
+
 namespace kci {
 // record a message for kc_last_error() and return s
 kc_status fail(kc_status s, const char *fmt, ...);
@@ -33,6 +35,9 @@ struct kc_ctx {
     uint32_t num_cus = 0;
     uint64_t hbm = 0;
     uint64_t clock_khz = 100000; // wall_clock64 rate
+    // the creator's reference plus one per graph or batch made on it: a
+    // context destroyed while they live stays until the last one is freed
+    std::atomic<int> refs{1};
 };
 
 struct DevBuf {
@@ -92,6 +97,9 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
 }
 
 namespace kci {
+// take / drop a reference to a context (the last drop destroys it)
+void ctx_ref(kc_ctx *c);
+void ctx_unref(kc_ctx *c);
 // grow b to at least `bytes` (contents are not kept)
 kc_status ensure(kc_graph *g, DevBuf &b, size_t bytes, const char *what);
 // run one method on the device; the result stays at g->result

@@ -3,13 +3,26 @@
 // bounded spin waits, and the edge-balanced batch walker.
 //
 // Memory-model notes (CDNA3/CDNA4, 8 XCDs with private L2s):
-//  * degree / bitmap updates use HIP's agent-scope relaxed atomics, which are
-//    coherent across XCDs;
-//  * plain loads of those arrays may return stale (larger) values between
-//    barriers; every use below is safe under staleness because degrees only
-//    fall and every decision that needs a fresh value goes through an atomic;
-//  * the grid barrier does release/acquire fences at agent scope, so loads
-//    issued after it see everything written before it.
+//  * degree / bitmap / histogram updates use HIP's agent-scope relaxed
+//    atomics, which are performed past the XCD-private L2s and are coherent
+//    across XCDs;
+//  * the rule every engine that uses grid_barrier<false> follows: each word
+//    that one workgroup writes and another reads -- degrees, claim bits,
+//    work-list slots, counters, reduction slots -- is written by an
+//    agent-scope atomic or atomic store and read by an atomic or atomic load
+//    (ld_relaxed*), never by a plain load or store.  The non-flushing barrier
+//    (grid_barrier<false>) only waits for each workgroup's own memory
+//    operations to complete (vm_drain) before it arrives; it does not write
+//    back or invalidate any L2, so a plain store would stay in its XCD's L2
+//    and a plain load could return a line cached before the barrier.  The
+//    read-only graph (rowptr, col) is the only data read with plain loads;
+//  * one planned exception: the peel's pre-check reads a degree with a plain
+//    load.  A stale value is never smaller than the true one (degrees only
+//    fall within a phase) and only decides whether to issue the atomic,
+//    whose returned value decides everything else;
+//  * grid_barrier<true> (agent-scope release/acquire fences, which write back
+//    and invalidate every XCD's L2) is for engines whose cross-workgroup data
+//    is written with plain stores.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -25,7 +25,8 @@ enum {
                           // bitmap, histograms, counters, marks)
     KC_ST_TAIL_LEVEL = 9, // peel: level at which the kernel stopped early, or ~0
     KC_ST_LIVE = 10,      // peel: vertices still live when the kernel ended
-    KC_ST_COUNT = 11
+    KC_ST_REBUILDS = 11,  // HistoCore: rounds that recounted every hub's bins
+    KC_ST_COUNT = 12
 };
 
 // Optional per-phase trace (profiling aid), KC_TRACE_WORDS u64 per phase:
@@ -191,6 +192,21 @@ struct BuildOut {
     std::vector<uint64_t> edge_off; // host, G+1
 };
 hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std::string &err);
+
+// CSR of an edge file's kept pairs (host arrays, ids < V) built on the
+// device: keep_order (the COO reader) gives both directed entries of every
+// pair, duplicates kept, each row in file order; otherwise (the
+// deduplicating reader) the symmetric, deduplicated CSR with sorted rows.
+// rowptr is u64 when wide, else u32; both outputs are owned by the caller.
+struct PairCsr {
+    void *rowptr = nullptr;
+    uint32_t *col = nullptr;
+    uint64_t nnz = 0;
+    bool wide = false;
+};
+hipError_t build_csr_pairs(const uint32_t *h_src, const uint32_t *h_dst, uint64_t m, uint32_t V,
+                           bool keep_order, bool wide, hipStream_t st, PairCsr &out,
+                           std::string &err);
 
 // Core maintenance under edge updates (kc_dyn.hip) on a slack CSR: vertex v
 // owns slots [off[v], off[v+1]) of adj, the first len[v] of them in use.

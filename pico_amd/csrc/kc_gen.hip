@@ -377,6 +377,178 @@ hipError_t build_csr_device(const GenSpec &s, hipStream_t st, BuildOut &out, std
     return hipSuccess;
 }
 
+// ------------------------------------------------- edge-file pairs -> CSR
+// Both directed entries of every kept pair, keyed by their source row, in
+// file order: entry 2i is (src_i -> dst_i), entry 2i+1 (dst_i -> src_i).
+__global__ void expand_pairs32(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                               uint32_t *key, uint32_t *val)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        uint32_t a = src[i], b = dst[i];
+        key[2 * i] = a;
+        val[2 * i] = b;
+        key[2 * i + 1] = b;
+        val[2 * i + 1] = a;
+    }
+}
+
+__global__ void expand_pairs64(const uint32_t *src, const uint32_t *dst, uint64_t m,
+                               uint64_t *keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        uint64_t a = src[i], b = dst[i];
+        keys[2 * i] = a == b ? KEY_NONE : (a << 32) | b;
+        keys[2 * i + 1] = a == b ? KEY_NONE : (b << 32) | a;
+    }
+}
+
+// Row starts [0, V] from n row keys sorted ascending (all < V).
+template <typename RP>
+__global__ void rows_from_sorted32(const uint32_t *keys, uint64_t n, uint32_t V, RP *rp)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        uint64_t k = i < n ? keys[i] : V;
+        uint64_t p = i > 0 ? (uint64_t)keys[i - 1] + 1 : 0;
+        for (uint64_t w = p; w <= k && w <= V; w++)
+            rp[w] = (RP)i;
+    }
+}
+
+__global__ void narrow_rows(const uint64_t *in, uint64_t n, uint32_t *out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (uint32_t)in[i];
+}
+
+hipError_t build_csr_pairs(const uint32_t *h_src, const uint32_t *h_dst, uint64_t m, uint32_t V,
+                           bool keep_order, bool wide, hipStream_t st, PairCsr &out,
+                           std::string &err)
+{
+    out = PairCsr{};
+    const uint64_t n2 = 2 * m;
+    if (n2 >= (1ull << 31)) {
+        err = "more than 2^30 edges: beyond the device reader's sort";
+        return hipErrorInvalidValue;
+    }
+    if (V == 0xffffffffu) {
+        err = "too many vertices";
+        return hipErrorInvalidValue;
+    }
+    Scratch sc;
+    uint32_t *d_src = nullptr, *d_dst = nullptr;
+    GEN_TRY(sc.get(&d_src, m));
+    GEN_TRY(sc.get(&d_dst, m));
+    if (m) {
+        GEN_TRY(hipMemcpyAsync(d_src, h_src, m * 4, hipMemcpyHostToDevice, st));
+        GEN_TRY(hipMemcpyAsync(d_dst, h_dst, m * 4, hipMemcpyHostToDevice, st));
+    }
+    const int nbk = (int)std::min<uint64_t>(8192, (n2 + 255) / 256 + 1);
+    void *rp = nullptr;
+    uint32_t *col = nullptr;
+    uint64_t nnz = 0;
+    struct Owned {
+        void *&rp;
+        uint32_t *&col;
+        bool keep = false;
+        ~Owned()
+        {
+            if (!keep) {
+                hipFree(rp);
+                hipFree(col);
+            }
+        }
+    } own{rp, col};
+    if (keep_order) {
+        // COO reader: duplicates kept, each row in file order -- a stable
+        // radix sort of the directed entries by source row
+        uint32_t *k0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+        GEN_TRY(sc.get(&k0, n2));
+        GEN_TRY(sc.get(&k1, n2));
+        GEN_TRY(hipMalloc((void **)&col, n2 ? n2 * 4 : 4));
+        GEN_TRY(sc.get(&v1, n2));
+        if (m) {
+            hipLaunchKernelGGL(expand_pairs32, dim3(nbk), dim3(256), 0, st, d_src, d_dst, m, k0,
+                               col);
+            GEN_TRY(hipGetLastError());
+            hipcub::DoubleBuffer<uint32_t> kb(k0, k1), vb(col, v1);
+            size_t need = 0;
+            const int end_bit = (int)bits_for(V);
+            GEN_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kb, vb, (int)n2, 0, end_bit,
+                                                       st));
+            void *tmp = nullptr;
+            GEN_TRY(sc.get((uint8_t **)&tmp, need));
+            GEN_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, need, kb, vb, (int)n2, 0, end_bit,
+                                                       st));
+            if (vb.Current() != col)
+                GEN_TRY(hipMemcpyAsync(col, vb.Current(), n2 * 4, hipMemcpyDeviceToDevice, st));
+            k0 = kb.Current();
+        }
+        nnz = n2;
+        GEN_TRY(hipMalloc(&rp, ((uint64_t)V + 1) * (wide ? 8 : 4)));
+        if (wide)
+            hipLaunchKernelGGL(rows_from_sorted32<uint64_t>, dim3(nbk), dim3(256), 0, st, k0, n2,
+                               V, (uint64_t *)rp);
+        else
+            hipLaunchKernelGGL(rows_from_sorted32<uint32_t>, dim3(nbk), dim3(256), 0, st, k0, n2,
+                               V, (uint32_t *)rp);
+        GEN_TRY(hipGetLastError());
+    } else {
+        // dedup reader: symmetric, deduplicated, rows sorted -- sort the
+        // (src << 32 | dst) keys of both directions and drop repeats
+        uint64_t *k0 = nullptr, *k1 = nullptr, *rp64 = nullptr;
+        int *d_nsel = nullptr;
+        GEN_TRY(sc.get(&k0, n2));
+        GEN_TRY(sc.get(&k1, n2));
+        GEN_TRY(sc.get(&d_nsel, 1));
+        GEN_TRY(sc.get(&rp64, (uint64_t)V + 1));
+        int nsel = 0;
+        uint64_t *uniq = k0;
+        if (m) {
+            hipLaunchKernelGGL(expand_pairs64, dim3(nbk), dim3(256), 0, st, d_src, d_dst, m, k0);
+            GEN_TRY(hipGetLastError());
+            hipcub::DoubleBuffer<uint64_t> db(k0, k1);
+            const int end_bit = (int)std::min<uint32_t>(64, 32 + bits_for(V));
+            size_t need = 0, need2 = 0;
+            GEN_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, need, db, (int)n2, 0, end_bit, st));
+            GEN_TRY(hipcub::DeviceSelect::Unique(nullptr, need2, k0, k1, d_nsel, (int)n2, st));
+            void *tmp = nullptr;
+            size_t tb = std::max(need, need2);
+            GEN_TRY(sc.get((uint8_t **)&tmp, tb));
+            GEN_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, need, db, (int)n2, 0, end_bit, st));
+            uint64_t *sorted = db.Current();
+            uniq = db.Alternate();
+            GEN_TRY(hipcub::DeviceSelect::Unique(tmp, tb, sorted, uniq, d_nsel, (int)n2, st));
+            GEN_TRY(hipMemcpyAsync(&nsel, d_nsel, 4, hipMemcpyDeviceToHost, st));
+            GEN_TRY(hipStreamSynchronize(st));
+        }
+        // a KEY_NONE (never produced: the reader drops self-loops) would sort last
+        nnz = (uint64_t)nsel;
+        GEN_TRY(hipMalloc((void **)&col, nnz ? nnz * 4 : 4));
+        hipLaunchKernelGGL(rows_from_keys, dim3(nbk), dim3(256), 0, st, uniq, nnz, V,
+                           (uint64_t)0, col, rp64);
+        GEN_TRY(hipGetLastError());
+        GEN_TRY(hipMalloc(&rp, ((uint64_t)V + 1) * (wide ? 8 : 4)));
+        if (wide) {
+            GEN_TRY(hipMemcpyAsync(rp, rp64, ((uint64_t)V + 1) * 8, hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL(narrow_rows, dim3(nbk), dim3(256), 0, st, rp64, (uint64_t)V + 1,
+                               (uint32_t *)rp);
+            GEN_TRY(hipGetLastError());
+        }
+    }
+    GEN_TRY(hipStreamSynchronize(st));
+    own.keep = true;
+    out.rowptr = rp;
+    out.col = col;
+    out.nnz = nnz;
+    out.wide = wide;
+    return hipSuccess;
+}
+
 // plain streaming copy (16 B per lane per step): the bandwidth probe the
 // HBM counters of the engines are scaled against
 __global__ void copy_probe_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,

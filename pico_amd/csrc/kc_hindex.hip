@@ -660,6 +660,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
     uint32_t gen = 0;
     uint32_t round = 0;
     uint64_t changed_total = 0;
+    uint64_t rebuilds = 0;  // HistoCore rounds that recounted the hubs
     uint64_t upd_edges = 0; // adjacency entries of changed vertices walked (HistoCore)
     uint64_t chg_deg = 0;   // degrees of changed vertices, summed (HistoCore)
     uint64_t n_atom = 0;    // atomics on graph-sized arrays (histograms, marks)
@@ -1110,6 +1111,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             const bool pull = n_hubs && a.nd_ds &&
                               (uint64_t)n_chunks * KC_HCH * KC_PULL_DEN >
                                   hub_edges * KC_PULL_NUM;
+            rebuilds += pull ? 1 : 0;
             if (pull) {
                 // Rebuild every hub's bins 1..core (bin 0 keeps the value
                 // the last collapse stored; bins above core are left as
@@ -1466,6 +1468,7 @@ out:
         a.stats[KC_ST_ITERS] = round;
         a.stats[KC_ST_PHASES] = gen;
         a.stats[KC_ST_CHANGED] = changed_total;
+        a.stats[KC_ST_REBUILDS] = rebuilds;
         a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
     }
     // per-workgroup sums first: one global atomic per counter and workgroup

@@ -35,6 +35,10 @@ struct Req {
     std::vector<uint32_t> u, v; // update edges, copied at submit
     double t_submit = 0;
     uint64_t bytes = 0;
+    // UPDATE / DECOMPOSE_GRAPH: the resident graph and its context, copied
+    // under the server lock at admission (the worker runs without the lock)
+    kc_graph *g = nullptr;
+    kc_ctx *gctx = nullptr;
 };
 
 struct Resident {
@@ -54,7 +58,7 @@ struct kc_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Req *> pending;
     std::deque<kc_completion> done;
-    std::vector<Resident> graphs;
+    std::deque<Resident> graphs; // read and extended only under mu
     uint64_t next_id = 1;
     uint64_t inflight_bytes = 0;
     uint32_t busy = 0;
@@ -141,7 +145,10 @@ std::vector<Req *> admit(kc_server *s)
         s->inflight_bytes += q->bytes;
         it = s->pending.erase(it);
         if (graph_kind(q->r)) {
-            s->graphs[q->r.graph].busy = true;
+            Resident &rg = s->graphs[q->r.graph];
+            rg.busy = true;
+            q->g = rg.g;
+            q->gctx = rg.ctx;
             break;
         }
         if (!small)
@@ -234,8 +241,7 @@ void run_job(kc_server *s, kc_ctx *ctx, std::vector<Req *> &job, std::vector<kc_
         c.kmax = (uint32_t)ks.kmax;
         kc_graph_free(g);
     } else {
-        Resident &rg = s->graphs[q0->r.graph];
-        kc_graph *g = rg.g;
+        kc_graph *g = q0->g;
         if (q0->r.kind == KC_REQ_UPDATE) {
             kc_update_stats us{};
             st = kc_update(g, (kc_update_op)q0->r.op, q0->u.data(), q0->v.data(), q0->u.size(),
@@ -246,7 +252,7 @@ void run_job(kc_server *s, kc_ctx *ctx, std::vector<Req *> &job, std::vector<kc_
             st = kc_decompose(g, KC_PEELONE_DYN, nullptr, q0->r.core_out, &ks);
         }
         if (st == KC_OK)
-            st = checksum_of(g->result, g->V, rg.ctx->stream, c.checksum);
+            st = checksum_of(g->result, g->V, q0->gctx->stream, c.checksum);
         c.V = g->V;
         c.kmax = 0;
     }

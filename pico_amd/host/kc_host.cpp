@@ -12,6 +12,11 @@
 #include <thread>
 #include <unordered_set>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 namespace kch {
 
 unsigned worker_threads()
@@ -310,20 +315,56 @@ uint64_t core_checksum(const uint32_t *core, uint32_t V)
 }
 
 // ------------------------------------------------------------- readers -----
-static bool slurp(const char *path, std::vector<char> &buf, std::string &err)
+// The file image: memory-mapped where the file allows it, so that the parse
+// threads fault its pages in side by side; read into memory otherwise.
+struct FileImage {
+    const char *p = nullptr;
+    size_t n = 0;
+    void *map = nullptr;
+    std::vector<char> buf;
+    FileImage() = default;
+    FileImage(const FileImage &) = delete;
+    FileImage &operator=(const FileImage &) = delete;
+    ~FileImage()
+    {
+        if (map)
+            munmap(map, n);
+    }
+};
+
+static bool open_image(const char *path, FileImage &im, std::string &err)
 {
-    FILE *f = std::fopen(path, "rb");
-    if (!f) {
+    int fd = ::open(path, O_RDONLY);
+    if (fd < 0) {
         err = std::string("cannot open ") + path;
         return false;
     }
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    buf.resize((size_t)(sz > 0 ? sz : 0) + 1);
-    size_t rd = sz > 0 ? std::fread(buf.data(), 1, (size_t)sz, f) : 0;
-    std::fclose(f);
-    buf.resize(rd);
+    struct stat sb;
+    if (fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0) {
+        void *m = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m != MAP_FAILED) {
+            ::close(fd);
+            im.map = m;
+            im.p = (const char *)m;
+            im.n = (size_t)sb.st_size;
+            return true;
+        }
+    }
+    char tmp[1 << 16];
+    for (;;) {
+        ssize_t r = ::read(fd, tmp, sizeof tmp);
+        if (r < 0) {
+            ::close(fd);
+            err = std::string("cannot read ") + path;
+            return false;
+        }
+        if (r == 0)
+            break;
+        im.buf.insert(im.buf.end(), tmp, tmp + r);
+    }
+    ::close(fd);
+    im.p = im.buf.data();
+    im.n = im.buf.size();
     return true;
 }
 
@@ -336,13 +377,14 @@ static inline bool is_ws(char c)
 // it in the C++11 library: skip whitespace; at end of line the stream fails
 // and the target keeps its old value; a field without digits stores 0 and
 // fails; '-' negates modulo 2^32; overflow stores UINT_MAX and fails.
-// Returns true when the stream is still good afterwards.
-static bool extract_u32(const char *&p, const char *end, uint32_t &val)
+// Returns 0 (target untouched, stream failed), 1 (target written, stream
+// failed) or 2 (target written, stream still good).
+static int extract_u32(const char *&p, const char *end, uint32_t &val)
 {
     while (p < end && is_ws(*p))
         p++;
     if (p >= end)
-        return false; // sentry fails: value untouched
+        return 0; // sentry fails: value untouched
     bool neg = false;
     if (*p == '-' || *p == '+') {
         neg = *p == '-';
@@ -350,7 +392,7 @@ static bool extract_u32(const char *&p, const char *end, uint32_t &val)
     }
     if (p >= end || *p < '0' || *p > '9') {
         val = 0;
-        return false;
+        return 1;
     }
     uint64_t x = 0;
     bool ovf = false;
@@ -364,71 +406,306 @@ static bool extract_u32(const char *&p, const char *end, uint32_t &val)
     }
     if (ovf) {
         val = 0xffffffffu;
-        return false;
+        return 1;
     }
     val = neg ? (uint32_t)(0u - (uint32_t)x) : (uint32_t)x;
-    return true;
+    return 2;
 }
 
-// Line splitter shared by both readers: calls fn(line_begin, line_end) for
-// every line after the first two (both reference readers discard exactly
-// two lines: shared/graph.cpp:47-48, src/graph.cpp:78-79).
-template <typename F> static void for_data_lines(const std::vector<char> &buf, F &&fn)
+// What one data line does to the reader's two carried values.  Both
+// reference readers extract the source id, and the target id only when that
+// succeeded (shared/graph.cpp:50-55, src/graph.cpp:82-88); a value the line
+// does not write keeps the previous line's.
+struct LineFx {
+    uint32_t vs = 0, vt = 0;
+    uint8_t ws = 0, wt = 0; // the line writes s / t
+};
+
+static inline LineFx line_fx(const char *p, const char *le)
 {
-    const char *p = buf.data(), *end = p + buf.size();
-    int skipped = 0;
+    LineFx f;
+    int r = extract_u32(p, le, f.vs);
+    f.ws = r != 0;
+    if (r == 2)
+        f.wt = extract_u32(p, le, f.vt) != 0;
+    return f;
+}
+
+// The line loop of both readers, run on worker threads over chunks of whole
+// lines, with results identical to one pass over the file.
+//  * COO reader (dedup = false): s, t carry over as written; a line keeps
+//    the edge (s, t) unless s == t.
+//  * dedup reader (dedup = true): after the extraction both values are
+//    decremented on every line, written or not (src/graph.cpp:89-90), so a
+//    value carried over is decremented again.
+// A chunk's lines until both values have been written by a line of the chunk
+// depend on the state the previous chunk leaves; they are kept as line
+// effects and replayed, chunk by chunk in file order, once every chunk is
+// parsed (usually the chunk's first line writes both and nothing waits).
+struct ParsedEdges {
+    std::vector<uint32_t> es, ed; // kept edges in file order
+    uint32_t mx = 0;              // largest id of a kept edge
+    bool zero = false;            // some kept edge has a 0 id
+    unsigned threads = 1;
+};
+
+namespace {
+struct ChunkParse {
+    std::vector<LineFx> wait; // lines replayed with the incoming state
+    std::vector<uint32_t> es, ed;
+    std::vector<uint32_t> pes, ped; // edges of the replayed lines
+    bool ks = false, kt = false;
+    uint32_t s = 0, t = 0;
+    uint32_t mx = 0;
+    bool zero = false;
+};
+
+struct EdgeSink {
+    std::vector<uint32_t> &es, &ed;
+    uint32_t &mx;
+    bool &zero;
+    inline void line(uint32_t s, uint32_t t)
+    {
+        if (s == t)
+            return;
+        es.push_back(s);
+        ed.push_back(t);
+        zero |= (s == 0) | (t == 0);
+        uint32_t m = s > t ? s : t;
+        mx = m > mx ? m : mx;
+    }
+};
+} // namespace
+
+static inline void apply_fx(const LineFx &f, bool dedup, uint32_t &s, uint32_t &t)
+{
+    if (f.ws)
+        s = f.vs;
+    if (f.wt)
+        t = f.vt;
+    if (dedup) {
+        s -= 1;
+        t -= 1;
+    }
+}
+
+static void parse_chunk(const char *p, const char *end, bool dedup, ChunkParse &c)
+{
+    EdgeSink sink{c.es, c.ed, c.mx, c.zero};
     while (p < end) {
         const char *nl = (const char *)std::memchr(p, '\n', (size_t)(end - p));
         const char *le = nl ? nl : end;
-        if (skipped < 2)
-            skipped++;
-        else
-            fn(p, le);
+        LineFx f = line_fx(p, le);
+        if (c.ks && c.kt) {
+            apply_fx(f, dedup, c.s, c.t);
+            sink.line(c.s, c.t);
+        } else {
+            c.wait.push_back(f);
+            c.ks |= f.ws != 0;
+            c.kt |= f.wt != 0;
+            if (c.ks && c.kt) {
+                // the values after this line no longer depend on the
+                // incoming state: replay the waiting lines from any state
+                uint32_t s = 0, t = 0;
+                for (const LineFx &w : c.wait)
+                    apply_fx(w, dedup, s, t);
+                c.s = s;
+                c.t = t;
+            }
+        }
         p = nl ? nl + 1 : end;
     }
+}
+
+static bool parse_edges(const char *path, bool dedup, ParsedEdges &out, uint64_t *bytes,
+                        std::string &err)
+{
+    FileImage im;
+    if (!open_image(path, im, err))
+        return false;
+    if (bytes)
+        *bytes = im.n;
+    // both reference readers discard exactly two lines first
+    // (shared/graph.cpp:47-48, src/graph.cpp:78-79)
+    const char *p = im.p, *end = im.p + im.n;
+    for (int k = 0; k < 2 && p < end; k++) {
+        const char *nl = (const char *)std::memchr(p, '\n', (size_t)(end - p));
+        p = nl ? nl + 1 : end;
+    }
+    const uint64_t n = (uint64_t)(end - p);
+    unsigned T = serial() ? 1 : worker_threads();
+    // chunks of at least 1 MiB, a few per thread for balance; KC_PARSE_CHUNK
+    // (bytes) sets the chunk size itself, uncapped (tests cut small files
+    // into many chunks with it)
+    uint64_t nch;
+    const char *ce = std::getenv("KC_PARSE_CHUNK");
+    if (ce && std::strtoull(ce, nullptr, 10) > 0)
+        nch = std::max<uint64_t>(1, n / std::strtoull(ce, nullptr, 10));
+    else
+        nch = std::max<uint64_t>(1, std::min<uint64_t>(n >> 20, 4ull * T));
+    std::vector<const char *> cut(nch + 1);
+    cut[0] = p;
+    cut[nch] = end;
+    for (uint64_t k = 1; k < nch; k++) {
+        const char *q = p + n * k / nch;
+        if (q < cut[k - 1])
+            q = cut[k - 1];
+        const char *nl = (const char *)std::memchr(q, '\n', (size_t)(end - q));
+        cut[k] = nl ? nl + 1 : end;
+    }
+    std::vector<ChunkParse> ch(nch);
+    parallel_for(nch, 1, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; k++)
+            parse_chunk(cut[k], cut[k + 1], dedup, ch[k]);
+    });
+    out.threads = nch > 1 ? std::min<unsigned>(T, (unsigned)nch) : 1u;
+    // replay the waiting lines in file order; both readers start from s = t = 0
+    uint32_t s = 0, t = 0;
+    uint64_t total = 0;
+    for (auto &c : ch) {
+        EdgeSink sink{c.pes, c.ped, c.mx, c.zero};
+        for (const LineFx &w : c.wait) {
+            apply_fx(w, dedup, s, t);
+            sink.line(s, t);
+        }
+        if (c.ks && c.kt) {
+            s = c.s;
+            t = c.t;
+        }
+        total += c.pes.size() + c.es.size();
+        out.zero |= c.zero;
+        out.mx = std::max(out.mx, c.mx);
+    }
+    out.es.resize(total);
+    out.ed.resize(total);
+    std::vector<uint64_t> pos(nch + 1, 0);
+    for (uint64_t k = 0; k < nch; k++)
+        pos[k + 1] = pos[k] + ch[k].pes.size() + ch[k].es.size();
+    parallel_for(nch, 1, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; k++) {
+            ChunkParse &c = ch[k];
+            uint64_t o = pos[k];
+            size_t np = c.pes.size(), nm = c.es.size();
+            if (np) {
+                std::memcpy(out.es.data() + o, c.pes.data(), np * 4);
+                std::memcpy(out.ed.data() + o, c.ped.data(), np * 4);
+            }
+            if (nm) {
+                std::memcpy(out.es.data() + o + np, c.es.data(), nm * 4);
+                std::memcpy(out.ed.data() + o + np, c.ed.data(), nm * 4);
+            }
+            c = ChunkParse{};
+        }
+    });
+    return true;
+}
+
+// Vertex count and id shift of the COO reader (shared/graph.cpp:89-92,
+// virtual_graph.cpp:13-20): V = max id, plus one when some id is 0; without
+// a 0 id every id moves down by one.  Ids that would give 2^32 - 1 or more
+// vertices are refused (the reference would index past its arrays).
+static bool coo_vertices(const ParsedEdges &pe, uint32_t &V, std::string &err)
+{
+    if (pe.es.empty()) {
+        V = 0;
+        return true;
+    }
+    if (pe.zero ? pe.mx >= 0xfffffffeu : pe.mx == 0xffffffffu) {
+        err = "vertex id " + std::to_string(pe.mx) + " is too large (at most 2^32 - 2 vertices)";
+        return false;
+    }
+    V = pe.mx + (pe.zero ? 1u : 0u);
+    return true;
+}
+
+// Vertex count of the dedup reader: max id + 1 after the shift (src/graph.cpp:102).
+static bool dedup_vertices(const ParsedEdges &pe, uint32_t &V, std::string &err)
+{
+    if (pe.es.empty()) {
+        V = 1; // V++ of an empty maximum
+        return true;
+    }
+    if (pe.mx == 0xffffffffu) {
+        err = "the deduplicating reader needs 1-based vertex ids (found id 0)";
+        return false;
+    }
+    if (pe.mx >= 0xfffffffeu) {
+        err = "vertex id " + std::to_string((uint64_t)pe.mx + 1) +
+              " is too large (at most 2^32 - 2 vertices)";
+        return false;
+    }
+    V = pe.mx + 1u;
+    return true;
 }
 
 // Reader of gpp / peelone / HistoCore / CntCore / NbrCore
 // (shared/graph.cpp:17-99 + virtual_graph.cpp:9-43, 93-157): keeps duplicate
 // edges, drops self-loops, num_nodes = max id (+1 when some id is 0), shifts
 // to 0-based when no id is 0, neighbours in edge-insertion order.
+int read_coo_pairs(const char *path, EdgeFile &ef, std::string &err)
+{
+    ParsedEdges pe;
+    if (!parse_edges(path, false, pe, &ef.file_bytes, err))
+        return -1;
+    uint32_t V = 0;
+    if (!coo_vertices(pe, V, err))
+        return -2;
+    if (!pe.zero) {
+        parallel_for(pe.es.size(), 1 << 20, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++) {
+                pe.es[i]--;
+                pe.ed[i]--;
+            }
+        });
+    }
+    ef.V = V;
+    ef.has_zero_id = pe.zero;
+    ef.threads = pe.threads;
+    ef.src.swap(pe.es);
+    ef.dst.swap(pe.ed);
+    return 0;
+}
+
+int read_dedup_pairs(const char *path, EdgeFile &ef, std::string &err)
+{
+    ParsedEdges pe;
+    if (!parse_edges(path, true, pe, &ef.file_bytes, err))
+        return -1;
+    uint32_t V = 0;
+    if (!dedup_vertices(pe, V, err))
+        return -2;
+    ef.V = V;
+    ef.has_zero_id = false;
+    ef.threads = pe.threads;
+    ef.src.swap(pe.es);
+    ef.dst.swap(pe.ed);
+    return 0;
+}
+
 int read_coo(const char *path, kc_host_graph &g, std::string &err)
 {
-    std::vector<char> buf;
-    if (!slurp(path, buf, err))
-        return -1;
-    std::vector<uint32_t> es, ed;
-    uint32_t s = 0, t = 0, mx = 0;
-    bool zero = false;
-    for_data_lines(buf, [&](const char *p, const char *le) {
-        if (extract_u32(p, le, s))
-            extract_u32(p, le, t);
-        if (s == t)
-            return;
-        es.push_back(s);
-        ed.push_back(t);
-        zero |= (s == 0) | (t == 0);
-        mx = std::max(mx, std::max(s, t));
-    });
-    uint64_t m = es.size();
-    uint32_t V = mx + (zero ? 1u : 0u);
-    if (!zero)
-        for (uint64_t i = 0; i < m; i++) {
-            es[i]--;
-            ed[i]--;
-        }
+    EdgeFile ef;
+    int r = read_coo_pairs(path, ef, err);
+    if (r)
+        return r;
+    const uint32_t V = ef.V;
+    const uint64_t m = ef.src.size();
+    const uint32_t *es = ef.src.data(), *ed = ef.dst.data();
     g.V = V;
     g.edges_read = m;
-    g.has_zero_id = zero;
+    g.has_zero_id = ef.has_zero_id;
     g.multigraph = true;
     g.rowptr.assign((size_t)V + 1, 0);
-    for (uint64_t i = 0; i < m; i++) {
-        g.rowptr[es[i] + 1]++;
-        g.rowptr[ed[i] + 1]++;
-    }
+    parallel_for(m, 1 << 20, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) {
+            __atomic_fetch_add(&g.rowptr[es[i] + 1], 1, __ATOMIC_RELAXED);
+            __atomic_fetch_add(&g.rowptr[ed[i] + 1], 1, __ATOMIC_RELAXED);
+        }
+    });
     for (uint32_t v = 0; v < V; v++)
         g.rowptr[v + 1] += g.rowptr[v];
     g.col.resize(g.rowptr[V]);
+    // rows in edge-insertion order: one pass over the edges in file order
     std::vector<uint64_t> cur(g.rowptr.begin(), g.rowptr.end() - 1);
     for (uint64_t i = 0; i < m; i++) {
         g.col[cur[es[i]]++] = ed[i];
@@ -444,35 +721,31 @@ int read_coo(const char *path, kc_host_graph &g, std::string &err)
 // reference and index out of bounds; they are rejected here.
 int read_dedup(const char *path, kc_host_graph &g, std::string &err)
 {
-    std::vector<char> buf;
-    if (!slurp(path, buf, err))
-        return -1;
-    std::vector<uint32_t> es, ed;
-    uint32_t s = 0, t = 0, val = 0, V = 0;
-    uint64_t lines = 0;
-    for_data_lines(buf, [&](const char *p, const char *le) {
-        if (extract_u32(p, le, s))
-            if (extract_u32(p, le, t))
-                extract_u32(p, le, val);
-        s = s - 1;
-        t = t - 1;
-        if (s == t)
-            return;
-        V = std::max(V, std::max(s, t));
-        es.push_back(s);
-        ed.push_back(t);
-        lines++;
-    });
-    if (V == 0xffffffffu) {
-        err = "the deduplicating reader needs 1-based vertex ids (found id 0)";
-        return -1;
-    }
-    uint32_t nv = V + 1u; // V++ of src/graph.cpp:102
-    int r = csr_from_pairs(nv, es.size(), es.data(), ed.data(), g);
-    g.edges_read = lines;
+    EdgeFile ef;
+    int r = read_dedup_pairs(path, ef, err);
+    if (r)
+        return r;
+    r = csr_from_pairs(ef.V, ef.src.size(), ef.src.data(), ef.dst.data(), g);
+    g.edges_read = ef.src.size();
     return r;
 }
 
+static inline char *put_u32(char *o, uint32_t x)
+{
+    char tmp[10];
+    int k = 0;
+    do {
+        tmp[k++] = (char)('0' + x % 10);
+        x /= 10;
+    } while (x);
+    while (k)
+        *o++ = tmp[--k];
+    return o;
+}
+
+// example.txt dialect: the two header lines, then "u v 1" (1-based) once per
+// undirected edge from its smaller endpoint, rows in order.  Rows are
+// formatted on worker threads in blocks and written in order.
 int write_edgelist(const kc_host_graph &g, const char *path, std::string &err)
 {
     FILE *f = std::fopen(path, "wb");
@@ -480,30 +753,59 @@ int write_edgelist(const kc_host_graph &g, const char *path, std::string &err)
         err = std::string("cannot create ") + path;
         return -1;
     }
+    const uint32_t V = g.V;
+    constexpr uint32_t ROWS = 1u << 16; // rows per formatted block
+    const uint32_t nblk = (V + ROWS - 1) / ROWS;
+    std::vector<uint64_t> half(nblk, 0);
+    parallel_for(nblk, 1, [&](uint64_t b, uint64_t e) {
+        for (uint64_t k = b; k < e; k++) {
+            uint64_t m = 0;
+            uint32_t v1 = (uint32_t)std::min<uint64_t>(V, (k + 1) * ROWS);
+            for (uint32_t v = (uint32_t)(k * ROWS); v < v1; v++)
+                for (uint64_t x = g.rowptr[v]; x < g.rowptr[v + 1]; x++)
+                    m += g.col[x] > v;
+            half[k] = m;
+        }
+    });
     uint64_t m = 0;
-    for (uint32_t v = 0; v < g.V; v++)
-        for (uint64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; e++)
-            m += g.col[e] > v;
-    std::fprintf(f, "%%%%MatrixMarket matrix coordinate integer symmetric\n%u %u %llu\n", g.V, g.V,
+    for (uint64_t h : half)
+        m += h;
+    std::fprintf(f, "%%%%MatrixMarket matrix coordinate integer symmetric\n%u %u %llu\n", V, V,
                  (unsigned long long)m);
-    std::vector<char> out;
-    out.reserve(1 << 22);
-    char tmp[32];
-    for (uint32_t v = 0; v < g.V; v++) {
-        for (uint64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; e++) {
-            uint32_t u = g.col[e];
-            if (u <= v)
-                continue;
-            int k = std::snprintf(tmp, sizeof tmp, "%u %u 1\n", v + 1, u + 1);
-            out.insert(out.end(), tmp, tmp + k);
-        }
-        if (out.size() > (1 << 22) - 64) {
-            std::fwrite(out.data(), 1, out.size(), f);
-            out.clear();
-        }
+    bool ok = true;
+    const uint32_t T = std::max(1u, serial() ? 1u : worker_threads());
+    std::vector<std::vector<char>> out(T);
+    for (uint32_t k0 = 0; k0 < nblk && ok; k0 += T) {
+        uint32_t nb = std::min<uint32_t>(T, nblk - k0);
+        parallel_for(nb, 1, [&](uint64_t b, uint64_t e) {
+            for (uint64_t j = b; j < e; j++) {
+                uint64_t k = k0 + j;
+                std::vector<char> &o = out[j];
+                o.resize(half[k] * 24 + 1);
+                char *w = o.data();
+                uint32_t v1 = (uint32_t)std::min<uint64_t>(V, (k + 1) * ROWS);
+                for (uint32_t v = (uint32_t)(k * ROWS); v < v1; v++)
+                    for (uint64_t x = g.rowptr[v]; x < g.rowptr[v + 1]; x++) {
+                        uint32_t u = g.col[x];
+                        if (u <= v)
+                            continue;
+                        w = put_u32(w, v + 1);
+                        *w++ = ' ';
+                        w = put_u32(w, u + 1);
+                        *w++ = ' ';
+                        *w++ = '1';
+                        *w++ = '\n';
+                    }
+                o.resize((size_t)(w - o.data()));
+            }
+        });
+        for (uint32_t j = 0; j < nb && ok; j++)
+            ok = std::fwrite(out[j].data(), 1, out[j].size(), f) == out[j].size();
     }
-    std::fwrite(out.data(), 1, out.size(), f);
-    std::fclose(f);
+    if (std::fclose(f) != 0 || !ok) {
+        err = std::string("cannot write ") + path;
+        return -1;
+    }
     return 0;
 }
 

@@ -113,6 +113,19 @@ int batch_from_graphs(std::vector<kc_host_graph> &gs, kc_host_batch &out, bool r
 // the device checksum of a core vector, computed on the host
 uint64_t core_checksum(const uint32_t *core, uint32_t V);
 
+// An edge file's kept edges in file order, after the reader's id shift.
+struct EdgeFile {
+    uint32_t V = 0;
+    std::vector<uint32_t> src, dst;
+    bool has_zero_id = false;
+    uint64_t file_bytes = 0;
+    unsigned threads = 1; // parse threads used
+};
+// The line loops of the two readers, parsed on worker threads (results as
+// one sequential pass); 0 ok, -1 file error, -2 malformed (ids out of range).
+int read_coo_pairs(const char *path, EdgeFile &ef, std::string &err);
+int read_dedup_pairs(const char *path, EdgeFile &ef, std::string &err);
+
 int read_coo(const char *path, kc_host_graph &g, std::string &err);
 int read_dedup(const char *path, kc_host_graph &g, std::string &err);
 int write_edgelist(const kc_host_graph &g, const char *path, std::string &err);

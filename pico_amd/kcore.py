@@ -9,8 +9,11 @@ Every call goes to the HIP library; there is no CPU fallback.  A missing or
 unloadable libkcore.so raises ``KCoreError`` at import of this module's first
 use.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -49,7 +52,7 @@ class Stats(ctypes.Structure):
                 ("stages", ctypes.c_uint32), ("device_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("changed", ctypes.c_uint64),
                 ("prep_ms", ctypes.c_double), ("changed_degree", ctypes.c_uint64),
-                ("atomics", ctypes.c_uint64)]
+                ("atomics", ctypes.c_uint64), ("rebuild_rounds", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_
@@ -61,6 +64,18 @@ class GraphInfo(ctypes.Structure):
                 ("edges_read", ctypes.c_uint64), ("has_zero_id", ctypes.c_uint32),
                 ("multigraph", ctypes.c_uint32), ("rowptr", ctypes.POINTER(ctypes.c_uint64)),
                 ("col", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class ReadStats(ctypes.Structure):
+    _fields_ = [("V", ctypes.c_uint32), ("has_zero_id", ctypes.c_uint32),
+                ("multigraph", ctypes.c_uint32), ("threads", ctypes.c_uint32),
+                ("nnz", ctypes.c_uint64), ("edges_read", ctypes.c_uint64),
+                ("file_bytes", ctypes.c_uint64), ("parse_ms", ctypes.c_double),
+                ("build_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class RefGraph(ctypes.Structure):
@@ -232,6 +247,8 @@ def lib():
                                   ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
                                   ctypes.POINTER(_vp)], S),
         "kc_graph_download": ([_vp, ctypes.POINTER(_vp)], S),
+        "kc_graph_read": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
+                           ctypes.POINTER(_vp), ctypes.POINTER(ReadStats)], S),
         "kc_batch_gen_rmat": ([_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
@@ -267,6 +284,41 @@ def lib():
     return L
 
 
+# Every live handle object, closed in dependency order when the interpreter
+# exits (servers, then device objects, then host objects, then contexts),
+# while the HIP runtime is still up; finalizers run during interpreter
+# shutdown make no library calls at all.
+_live = weakref.WeakSet()
+_CLOSE_ORDER = ("Server", "DeviceBatch", "DeviceGraph", "HostBatch", "HostGraph", "Context")
+
+
+def _register(obj):
+    _live.add(obj)
+
+
+def _close_all():
+    objs = list(_live)
+    for name in _CLOSE_ORDER:
+        for o in objs:
+            if type(o).__name__ == name:
+                try:
+                    o.close()
+                except Exception:
+                    pass
+
+
+atexit.register(_close_all)
+
+
+def _finalize(obj):
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
 def _check(status):
     if status != 0:
         raise KCoreError(status, lib().kc_last_error().decode(errors="replace"))
@@ -290,6 +342,7 @@ class HostGraph:
 
     def __init__(self, handle):
         self._h = handle
+        _register(self)
         info = GraphInfo()
         _check(lib().kc_hg_info(self._h, ctypes.byref(info)))
         self.V = int(info.V)
@@ -335,10 +388,7 @@ class HostGraph:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     @staticmethod
     def _make(fn, *args):
@@ -404,6 +454,7 @@ class HostBatch:
 
     def __init__(self, handle):
         self._h = handle
+        _register(self)
         info = BatchInfo()
         _check(lib().kc_hb_info(self._h, ctypes.byref(info)))
         self.G = int(info.G)
@@ -452,10 +503,7 @@ class HostBatch:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     @classmethod
     def rmat(cls, G, scale_min=8, scale_max=18, ef_min=4.0, ef_max=32.0, a=0.57, b=0.19,
@@ -475,9 +523,13 @@ class HostBatch:
 
 # ---------------------------------------------------------------- device
 class Context:
+    """kc_ctx: a device and its stream.  Graphs and batches made on it hold
+    a reference, so closing the context while they live is safe."""
+
     def __init__(self, device=0):
         self._h = _vp()
         _check(lib().kc_ctx_create(device, ctypes.byref(self._h)))
+        _register(self)
         cus = ctypes.c_uint32(0)
         hbm = ctypes.c_uint64(0)
         _check(lib().kc_ctx_info(self._h, ctypes.byref(cus), ctypes.byref(hbm)))
@@ -488,6 +540,9 @@ class Context:
         if self._h:
             lib().kc_ctx_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        _finalize(self)
 
     def upload(self, graph, rowptr_bytes=4):
         h = _vp()
@@ -514,6 +569,17 @@ class Context:
         _check(lib().kc_graph_gen_chunglu(self._h, n, beta, mean_deg, cap, seed, rowptr_bytes,
                                           ctypes.byref(h)))
         return DeviceGraph(self, h, n)
+
+    def read(self, path, reader="coo", rowptr_bytes=4):
+        """An edge file straight into a resident graph (kc_graph_read): the
+        HostGraph.read_coo / read_dedup graph, parsed on host threads and
+        built on the device.  Returns (DeviceGraph, stats dict)."""
+        r = {"coo": 0, "dedup": 1}[reader] if isinstance(reader, str) else int(reader)
+        h = _vp()
+        st = ReadStats()
+        _check(lib().kc_graph_read(self._h, os.fsencode(path), r, rowptr_bytes,
+                                   ctypes.byref(h), ctypes.byref(st)))
+        return DeviceGraph(self, h, int(st.V)), st.as_dict()
 
     def gen_rmat_batch(self, G, scale_min=8, scale_max=18, ef_min=4.0, ef_max=32.0, a=0.57,
                        b=0.19, c=0.19, seed=1):
@@ -570,6 +636,7 @@ class DeviceGraph:
         self.ctx = ctx
         self._h = handle
         self.V = V
+        _register(self)
 
     def download(self):
         h = _vp()
@@ -618,10 +685,7 @@ class DeviceGraph:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class DeviceBatch:
@@ -646,6 +710,7 @@ class DeviceBatch:
                                          _ptr(cl), ctypes.byref(h)))
             self.G, self.vertices = len(vo) - 1, int(vo[-1])
         self._h = h
+        _register(self)
 
     @property
     def device_bytes(self):
@@ -692,10 +757,7 @@ class DeviceBatch:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def core_checksum(core):
@@ -716,6 +778,7 @@ class Server:
                                                                 fuse_max_scale, hbm_budget)
         self._h = _vp()
         _check(lib().kc_server_create(device, ctypes.byref(o), ctypes.byref(self._h)))
+        _register(self)
         self._keep = {}
 
     def gen_rmat_graph(self, scale, edge_factor=16.0, seed=1):
@@ -789,10 +852,7 @@ class Server:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def vgraph_decompose(node_pointer, edge_list, method, **opts):

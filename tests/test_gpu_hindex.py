@@ -33,7 +33,7 @@ def _check(ctx, rowptr, col, method, width=4, hist=False, **opts):
         if hist:
             dev_hist = device_histogram(dg, hg.nnz + hg.V)
     finally:
-        pass
+        dg.close()
     want = oracle.bz(rowptr, col)
     np.testing.assert_array_equal(core, want)
     if method == "histocore":
@@ -44,7 +44,6 @@ def _check(ctx, rowptr, col, method, width=4, hist=False, **opts):
         ref_core, it = HIDX[method](rowptr, col)
     _, jac = oracle.hindex_jacobi(rowptr, col)
     assert st["iterations"] == it == jac, (st["iterations"], it, jac)
-    dg.close()
     return core, st
 
 
@@ -153,4 +152,32 @@ def test_histocore_hub_rebuild_paths(ctx, n, block, grid):
     # and 2099 (above a 256-thread group's LDS bins) to HBM bins; bin 0 and
     # the bins above the core must come out as the reference leaves them
     rp, col = _clique_with_leaves(n, 10)
-    _check(ctx, rp, col, "histocore", hist=True, block=block, grid=grid)
+    _, st = _check(ctx, rp, col, "histocore", hist=True, block=block, grid=grid)
+    assert st["rebuild_rounds"] > 0, st
+
+
+def _union(*parts):
+    """Disjoint union of CSR graphs (vertex ids shifted part by part)."""
+    rps, cols, off = [np.zeros(1, np.uint64)], [], 0
+    for rp, col in parts:
+        rps.append(rp[1:] + rps[-1][-1])
+        cols.append(col.astype(np.uint32) + np.uint32(off))
+        off += len(rp) - 1
+    return np.concatenate(rps), np.concatenate(cols)
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_histocore_wave_rebuild_and_hub_aggregation(ctx, block):
+    # K_100 whose members carry 600 leaves each: hubs of 699 neighbours
+    # (above the 512 of a rebuild hub, at most 4,096) with core 99 (below the
+    # 512 LDS bins), recounted by one wave each when the first round's changed
+    # hubs reach all of the hub adjacency; beside it K_300 with 50 leaves
+    # each: 349 neighbours (257..512), whose bin updates are summed per
+    # workgroup in LDS instead of patched one atomic at a time
+    rp, col = _union(_clique_with_leaves(100, 600), _clique_with_leaves(300, 50))
+    deg = np.diff(rp)
+    assert ((deg > 512) & (deg <= 4096)).sum() == 100
+    assert ((deg > 256) & (deg <= 512)).sum() == 300
+    core, st = _check(ctx, rp, col, "histocore", hist=True, block=block)
+    assert st["rebuild_rounds"] > 0, st
+    assert int(core[:100].max()) == 99 and int(core[60100:60400].min()) == 299

@@ -1,0 +1,112 @@
+"""kc_graph_read: the edge-file readers straight into a resident graph, the
+CSR built on the device (a stable sort of the directed entries for the COO
+reader of shared/graph.cpp:17-99 + virtual_graph.cpp:93-157; a sort + unique
+for the deduplicating reader of src/graph.cpp:58-133).  The resident graph
+must equal, entry for entry, the host reader's graph (itself pinned against
+the reference readers in test_readers.py), on the reader quirk fixtures and
+on generated files up to R-MAT scale 20."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import pico_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+RDIR = os.path.join(HERE, "golden", "readers")
+FIXTURES = sorted(os.path.join(RDIR, f) for f in os.listdir(RDIR) if f.endswith(".txt")) + \
+    [os.path.join(HERE, "golden", "example.txt")]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pa.Context(0)
+    yield c
+    c.close()
+
+
+def _host(reader, path):
+    return pa.HostGraph.read_coo(path) if reader == "coo" else pa.HostGraph.read_dedup(path)
+
+
+def _same_as_host(ctx, reader, path, rowptr_bytes=4):
+    try:
+        hg = _host(reader, path)
+    except pa.KCoreError as e:
+        with pytest.raises(pa.KCoreError) as ei:
+            ctx.read(path, reader, rowptr_bytes)
+        assert ei.value.status == e.status
+        return None, None
+    dg, st = ctx.read(path, reader, rowptr_bytes)
+    try:
+        back = dg.download()
+        assert st["V"] == hg.V == back.V
+        assert st["nnz"] == hg.nnz
+        assert st["edges_read"] == hg.edges_read
+        assert bool(st["has_zero_id"]) == hg.has_zero_id
+        np.testing.assert_array_equal(back.rowptr, hg.rowptr)
+        np.testing.assert_array_equal(back.col, hg.col)
+    except BaseException:
+        dg.close()
+        raise
+    return dg, hg
+
+
+@pytest.mark.parametrize("reader", ["coo", "dedup"])
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_quirk_fixtures(ctx, reader, path):
+    dg, _ = _same_as_host(ctx, reader, path)
+    if dg is not None:
+        dg.close()
+
+
+@pytest.mark.parametrize("reader", ["coo", "dedup"])
+def test_multigraph_file_and_wide_rows(ctx, reader, tmp_path):
+    # duplicates, reverse duplicates, loops, blank and one-id lines
+    rng = np.random.default_rng(4)
+    lines = []
+    for _ in range(20000):
+        a, b = rng.integers(1, 3000, 2)
+        k = rng.integers(10)
+        lines.append("" if k == 0 else "%d" % a if k == 1 else "%d %d 1" % (a, b))
+    p = str(tmp_path / "multi.txt")
+    with open(p, "w") as f:
+        f.write("%%x\n3000 3000 20000\n" + "\n".join(lines) + "\n")
+    for width in (4, 8):
+        dg, hg = _same_as_host(ctx, reader, p, width)
+        core, _ = dg.decompose("peelone_dyn")
+        np.testing.assert_array_equal(core, oracle.bz(hg.rowptr, hg.col))
+        dg.close()
+
+
+@pytest.mark.parametrize("reader", ["coo", "dedup"])
+def test_rmat_scale20_file(ctx, reader, tmp_path):
+    g = pa.HostGraph.rmat(20, 16, seed=3)
+    p = str(tmp_path / "rmat20.txt")
+    g.write_edgelist(p)
+    print("[reader] R-MAT s20 file %.0f MB" % (os.path.getsize(p) / 1e6), flush=True)
+    dg, hg = _same_as_host(ctx, reader, p)
+    try:
+        if reader == "dedup":
+            # the file lists each undirected edge once, 1-based: the dedup
+            # reader gives back the generator's graph (up to trailing
+            # isolated ids, which num_nodes = max id drops)
+            V = hg.V
+            np.testing.assert_array_equal(hg.rowptr, g.rowptr[:V + 1])
+            np.testing.assert_array_equal(hg.col, g.col[:hg.nnz])
+        core, _ = dg.decompose("peelone_dyn")
+        np.testing.assert_array_equal(core, oracle.bz(hg.rowptr, hg.col))
+    finally:
+        dg.close()
+
+
+def test_missing_file_and_bad_reader(ctx):
+    with pytest.raises(pa.KCoreError) as ei:
+        ctx.read("/nonexistent/graph.txt", "coo")
+    assert ei.value.status == 6
+    with pytest.raises(pa.KCoreError) as ei:
+        ctx.read(FIXTURES[0], 7)
+    assert ei.value.status == 1
