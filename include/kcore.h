@@ -230,6 +230,27 @@ KC_API uint32_t kc_graph_trace(const kc_graph *g, uint64_t *out, uint32_t max_ph
  * n <= nnz + V words are copied. */
 KC_API kc_status kc_graph_histogram(const kc_graph *g, uint32_t *out, uint64_t n);
 
+/* ------------------------------------------------- no device allocation */
+/* No counterpart in the reference.  For callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * device allocation): kc_decompose's bucket peel (KC_PEEL_DYN /
+ * KC_PEELONE_DYN, one persistent launch, no residual compaction) on a CSR
+ * the caller holds in device memory on ctx's device -- rowptr u32[V+1],
+ * col u32[nnz] -- writing the V core numbers to the caller's device buffer
+ * core.  Every work buffer (claim bitmap, work lists, live-vertex list,
+ * barrier and statistics block) lives in one static device array of
+ * KC_NOALLOC_SCRATCH_BYTES on each GPU, held until the process ends; the
+ * call makes no device allocation.  A graph whose work buffers need more
+ * (about 20 bytes per vertex plus 16 per 1,024 adjacency entries: R-MAT
+ * scale 22 fits) is refused with KC_ENOMEM.  The array is shared by every
+ * call on one GPU: calls that overlap on one GPU race and must be
+ * serialized by the caller.  Cores equal kc_decompose's. */
+#define KC_NOALLOC_SCRATCH_BYTES (96u << 20)
+KC_API kc_status kc_decompose_noalloc(kc_ctx *ctx, uint32_t V, uint64_t nnz,
+                                      const uint32_t *rowptr, const uint32_t *col,
+                                      kc_method method, const kc_opts *opts, uint32_t *core,
+                                      kc_stats *stats);
+
 /* ------------------------------------------------- reference-shaped ABI */
 /* Mirrors the data members of the peelone_dyn Graph (peel/inc/graph.h:7-23). */
 typedef struct kc_ref_graph {

@@ -38,6 +38,7 @@ LIB_SOURCES = [
     os.path.join(CSRC, "kc_batch_capi.cpp"),
     os.path.join(CSRC, "kc_dyn_capi.cpp"),
     os.path.join(CSRC, "kc_server.cpp"),
+    os.path.join(CSRC, "kc_noalloc.cpp"),
     os.path.join(HOST, "kc_host.cpp"),
     os.path.join(HOST, "kc_cache.cpp"),
 ]

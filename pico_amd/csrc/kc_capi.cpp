@@ -71,6 +71,10 @@ void ctx_unref(kc_ctx *c)
     if (!c || c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1)
         return;
     hipSetDevice(c->device);
+    if (c->ev0)
+        hipEventDestroy(c->ev0);
+    if (c->ev1)
+        hipEventDestroy(c->ev1);
     if (c->stream)
         hipStreamDestroy(c->stream);
     delete c;
@@ -375,22 +379,33 @@ kc_status kc_ctx_create(int device, kc_ctx **out)
     if (device < 0 || device >= n)
         return fail(KC_EINVAL, "device %d not present (%d visible)", device, n);
     KC_HIP(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    KC_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     auto *c = new (std::nothrow) kc_ctx;
     if (!c)
         return fail(KC_ENOMEM, "host allocation");
     c->device = device;
-    hipDeviceProp_t prop;
-    KC_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->num_cus = (uint32_t)prop.multiProcessorCount;
     c->hbm = prop.totalGlobalMem;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->clock_khz = (uint64_t)khz;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&c->ev1);
     if (e != hipSuccess) {
+        if (c->ev0)
+            hipEventDestroy(c->ev0);
+        if (c->stream)
+            hipStreamDestroy(c->stream);
         delete c;
-        return hip_fail(e, "hipStreamCreate");
+        return hip_fail(e, "context stream / events");
     }
+    // the no-allocation entry point's scratch array, located once per device
+    // (a failure is reported by that entry point, not here)
+    noalloc_setup(device);
     *out = c;
     return KC_OK;
 }
@@ -547,7 +562,7 @@ const uint32_t *kc_graph_core_device(const kc_graph *g) { return g ? g->result :
 // grid; two of them sharing a device from different streams could each hold
 // CUs the other's unscheduled workgroups need.  One lock per device keeps
 // them apart (everything else still overlaps).
-static std::mutex &coop_mutex(int device)
+std::mutex &kci::coop_mutex(int device)
 {
     static std::mutex mu[64];
     return mu[(unsigned)device & 63];
@@ -583,7 +598,7 @@ __global__ __launch_bounds__(256) void kc_max_kernel(const uint32_t *a, uint32_t
 // R-MAT scales 12-20 and the ER configuration, the best width grows about
 // with sqrt(nnz): 32 up to ~0.5M entries, 64 at 1.8M, 128 at 7.6M, the full
 // grid from ~30M on.
-static uint32_t size_grid(uint64_t nnz)
+uint32_t kci::size_grid(uint64_t nnz)
 {
     double g = 48.0 * std::sqrt((double)nnz / 1e6);
     return g < 32.0 ? 32u : g > 65536.0 ? 65536u : (uint32_t)g;

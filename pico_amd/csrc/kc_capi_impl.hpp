@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <mutex>
 
 namespace kci {
 // record a message for kc_last_error() and return s
@@ -35,6 +36,7 @@ struct kc_ctx {
     uint32_t num_cus = 0;
     uint64_t hbm = 0;
     uint64_t clock_khz = 100000; // wall_clock64 rate
+    hipEvent_t ev0 = nullptr, ev1 = nullptr; // timing of calls that own no graph
     // the creator's reference plus one per graph or batch made on it: a
     // context destroyed while they live stays until the last one is freed
     std::atomic<int> refs{1};
@@ -97,6 +99,13 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
 }
 
 namespace kci {
+// one lock per device: persistent grid-barrier launches never share the CUs
+std::mutex &coop_mutex(int device);
+// default persistent grid width for a graph of nnz adjacency entries
+uint32_t size_grid(uint64_t nnz);
+// locate the static scratch array of kc_decompose_noalloc on the current
+// device (kc_noalloc.cpp); called once per context creation
+kc_status noalloc_setup(int device);
 // take / drop a reference to a context (the last drop destroys it)
 void ctx_ref(kc_ctx *c);
 void ctx_unref(kc_ctx *c);
