@@ -55,7 +55,24 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals per s")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["peel", "batch", "serve"], default=None)
+    ap.add_argument("--serve-sweep", default="160,640",
+                    help="further Poisson rates (per s) of the serving stream, comma-separated")
+    ap.add_argument("--no-methods", action="store_true",
+                    help="skip GPP / PeelOne / CntCore / NbrCore on the headline graph")
+    ap.add_argument("--no-chunglu", action="store_true")
+    ap.add_argument("--no-reader", action="store_true")
+    ap.add_argument("--only", choices=["peel", "batch", "serve", "methods", "chunglu",
+                                       "reader"], default=None)
+    ap.add_argument("--profile", default=None,
+                    help="profiling run: build the graph (--graph), run the copy probe, then "
+                         "decompose twice with this one method (or 'batch': the 4,096-graph "
+                         "batch twice at block 1024, one workgroup per CU); prints a short line")
+    ap.add_argument("--graph", choices=["rmat", "chunglu"], default="rmat")
+    ap.add_argument("--pmc-key", default=None,
+                    help="with --summarize-pmc: the entry of profiles/pmc_traffic.json to "
+                         "write (merged into the file)")
+    ap.add_argument("--pmc-kernel", default=None,
+                    help="with --summarize-pmc and --pmc-key: kernel name substring")
     ap.add_argument("--pmc-stages", type=int, default=1,
                     help="with --summarize-pmc: peel_kernel launches per plain-peel step "
                          "(launches_per_step of the profiled run)")
@@ -111,7 +128,7 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 3) if len(xs) else None
 
 
-def bench_serving(args):
+def bench_serving(args, rate=None, n=None):
     """Poisson request stream against kc_server: fresh R-MAT decompositions
     (scale uniform over 10..20, edge factor 16) and 10,000-edge update batches
     (uniform random insertions / deletions of present edges) on a resident
@@ -126,8 +143,9 @@ def bench_serving(args):
     import oracle
     import pico_amd as pa
 
+    rate = float(rate or args.serve_rate)
     rng = np.random.default_rng(args.seed + 77)
-    n = args.serve_requests
+    n = int(n or args.serve_requests)
     base_scale, base_ef, base_seed = 20, 16.0, args.seed + 5
     # the workload, fixed up front: kinds, arrival times, parameters
     srv = pa.Server(slots=args.serve_slots, fuse_max_scale=16)
@@ -149,7 +167,7 @@ def bench_serving(args):
             pick = del_pool[dp: dp + 10_000]
             dp += 10_000
             reqs.append(("update", "delete", src[pick].copy(), col[pick].copy()))
-    gaps = rng.exponential(1.0 / args.serve_rate, n)
+    gaps = rng.exponential(1.0 / rate, n)
     arrive = np.cumsum(gaps) - gaps[0]
     done = {}
 
@@ -200,8 +218,8 @@ def bench_serving(args):
     return {
         "workload": "Poisson arrivals at %.1f/s: 50%% fresh R-MAT decompositions (scale uniform "
                     "10..20, edge factor 16), 50%% 10,000-edge insert/delete batches on a "
-                    "resident R-MAT s20 ef16 graph" % args.serve_rate,
-        "requests": n, "rate_per_s": args.serve_rate, "slots": args.serve_slots,
+                    "resident R-MAT s20 ef16 graph" % rate,
+        "requests": n, "rate_per_s": rate, "slots": args.serve_slots,
         "requests_per_s": round(n / ((last - first) * 1e-3), 2),
         "p50_ms": _pct(list(lat.values()), 50), "p99_ms": _pct(list(lat.values()), 99),
         "decompose_p50_ms": _pct(dec, 50), "decompose_p99_ms": _pct(dec, 99),
@@ -212,6 +230,160 @@ def bench_serving(args):
         "max_inflight_gb": round(st["max_inflight_bytes"] / 1e9, 3),
         "same_alone": "%d/%d" % (same, n), "final_cores_vs_oracle": final_ok,
     }
+
+
+def _timed(dg, method, reps, **opts):
+    """Warm run, then `reps` timed runs: (cores of the last, stats, mean device ms)."""
+    import numpy as np
+    dg.decompose(method, copy=False, **opts)
+    ms = []
+    for _ in range(reps):
+        _, st = dg.decompose(method, copy=False, **opts)
+        ms.append(st["device_ms"])
+    core, st = dg.decompose(method, **opts)
+    return core, st, float(np.mean(ms))
+
+
+def _roof(b_alg, kernel_ms, traffic=None):
+    a = b_alg / (kernel_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(a, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(a / HBM_PEAK_GBS, 5), "algorithmic_bytes": int(b_alg),
+           "traffic": traffic}
+    if traffic:
+        am = min(b_alg, traffic) / (kernel_ms * 1e-3) / 1e9
+        out["frac_min_rule"] = round(am / HBM_PEAK_GBS, 5)
+    return out
+
+
+PEEL_METHODS = ("gpp", "peelone", "peel_dyn", "peelone_dyn")
+
+
+def method_line(dg, method, V, nnz, m, R, reps, want=None, traffic_key=None, chg_deg=None,
+                **opts):
+    """One method on a resident graph: kernel time, GTEPS, and the roofline of
+    its algorithmic bytes (the peel model for the peel family, the h-index
+    model for the h-index family, both from the same cores / rounds)."""
+    core, st, kms = _timed(dg, method, reps, **opts)
+    if method in PEEL_METHODS:
+        b = peel_bytes(V, nnz, st["kmax"], R)
+        kern = "peel_kernel"
+    else:
+        # the three h-index methods run the same Jacobi sequence (same rounds,
+        # same changed vertices); HistoCore's kernel sums the changed degrees
+        cd = st["changed_degree"] if method == "histocore" or chg_deg is None else chg_deg
+        b = hindex_bytes(V, nnz, st["iterations"], st["changed"], cd, R)
+        kern = "hindex_kernel"
+    line = {"kernel_ms": round(kms, 4), "gteps": round(m / (kms * 1e-3) / 1e9, 4),
+            "iterations": st["iterations"], "kmax": st["kmax"], "stages": st["stages"],
+            "grid": st["grid"], "block": st["block"], "kernel": kern,
+            "roofline": _roof(b, kms, traffic_from_profile(traffic_key) if traffic_key
+                              else None)}
+    if method not in PEEL_METHODS:
+        line["rebuild_rounds"] = st.get("rebuild_rounds")
+        line["atomics"] = st["atomics"]
+        line["changed"] = st["changed"]
+    if want is not None:
+        import numpy as np
+        exp = want.copy()
+        if method == "gpp" and opts.get("gpp_compat"):
+            exp[exp == 0] = 1
+        line["parity_vs_oracle"] = bool(np.array_equal(core, exp))
+    return line
+
+
+def bench_methods(dg, V, nnz, m, want, chg_deg, reps=3):
+    """GPP, PeelOne, peel_dyn, CntCore and NbrCore on the headline graph (the
+    bucket peel and HistoCore have lines of their own)."""
+    return {mt: method_line(dg, mt, V, nnz, m, 4, reps, want, traffic_key=mt, chg_deg=chg_deg)
+            for mt in ("gpp", "peelone", "peel_dyn", "cntcore", "nbrcore")}
+
+
+def bench_chunglu(ctx, args, check=True):
+    """The heavy-skew Chung-Lu graph (n = 2^24, exponent 2.1, mean degree 20,
+    cap 4M, seed 3) with u64 row offsets: the bucket peel and HistoCore."""
+    import numpy as np
+
+    import oracle
+    t0 = time.time()
+    dg = ctx.gen_chunglu(1 << 24, 2.1, 20.0, 4e6, seed=3, rowptr_bytes=8)
+    gen_s = time.time() - t0
+    want = None
+    g = dg.download()
+    V, nnz = g.V, g.nnz
+    deg = np.diff(g.rowptr)
+    if check:
+        want = oracle.bz(g.rowptr, g.col)
+    out = {"workload": "Chung-Lu n=2^24, exponent 2.1, mean degree 20, max expected degree "
+                       "4M, seed 3; int32 ids, int64 row offsets, built in HBM",
+           "V": V, "undirected_edges": nnz // 2, "dmax": int(deg.max()),
+           "hubs_over_1e5": int((deg > 100000).sum()), "graph_gen_s": round(gen_s, 2)}
+    for mt in ("peelone_dyn", "histocore"):
+        out[mt] = method_line(dg, mt, V, nnz, nnz // 2, 8, max(2, args.steps // 3), want,
+                              traffic_key="chunglu_" + mt)
+    dg.close()
+    return out
+
+
+def bench_reader(ctx, g, args):
+    """The R-MAT s22 graph written as an edge file (example.txt dialect, one
+    line per undirected edge, 1-based) and read back straight into HBM by
+    kc_graph_read with both reference readers: wall time from the file (in the
+    page cache) to a resident graph, and equality with the generator's graph."""
+    import tempfile
+
+    import numpy as np
+
+    import pico_amd as pa
+    d = os.environ.get("TMPDIR") or tempfile.gettempdir()
+    path = os.path.join(d, "kc_bench_rmat%d_%d.txt" % (args.scale, os.getpid()))
+    out = {}
+    try:
+        t = time.perf_counter()
+        g.write_edgelist(path)
+        out["write_s"] = round(time.perf_counter() - t, 3)
+        out["file_bytes"] = os.path.getsize(path)
+        for reader in ("dedup", "coo"):
+            best = None
+            for _ in range(2):
+                rg, st = ctx.read(path, reader)
+                if best is None or st["total_ms"] < best[1]["total_ms"]:
+                    if best:
+                        best[0].close()
+                    best = (rg, st)
+                else:
+                    rg.close()
+            rg, st = best
+            back = rg.download()
+            rg.close()
+            if reader == "dedup":
+                # one line per undirected edge: the generator's graph back, up
+                # to trailing isolated ids (num_nodes = max id)
+                same = bool(np.array_equal(back.rowptr, g.rowptr[:back.V + 1]) and
+                            np.array_equal(back.col, g.col[:back.nnz]))
+            else:
+                # the COO reader keeps file order, which for this file (rows in
+                # order, each edge once from its smaller end) is sorted order
+                same = bool(np.array_equal(back.rowptr, g.rowptr[:back.V + 1]) and
+                            np.array_equal(back.col, g.col[:back.nnz]))
+            out[reader] = {"total_ms": round(st["total_ms"], 1),
+                           "parse_ms": round(st["parse_ms"], 1),
+                           "build_ms": round(st["build_ms"], 1), "threads": st["threads"],
+                           "V": st["V"], "nnz": st["nnz"], "matches_graph": same,
+                           "gb_per_s": round(out["file_bytes"] / (st["total_ms"] * 1e-3) / 1e9,
+                                             3)}
+        t = time.perf_counter()
+        hg = pa.HostGraph.read_dedup(path)
+        out["host_dedup_reader_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        hg.close()
+    finally:
+        try:
+            os.remove(path)
+        except OSError:
+            pass
+    out["workload"] = ("R-MAT scale %d edge factor %g seed %d written as a 1-based edge file, "
+                       "read into HBM (host threads parse, device builds the CSR)"
+                       % (args.scale, args.edge_factor, args.seed))
+    return out
 
 
 def bench_batched(ctx, args):
@@ -254,6 +426,11 @@ def bench_batched(ctx, args):
         ok = sum(ex.map(check, range(G)))
     cpu_s = time.perf_counter() - t
     ms = float(np.median(wall))
+    dev_med = float(np.median(dev))
+    Vg = np.diff(vo).astype(np.int64)
+    Eg = np.diff(eo).astype(np.int64)
+    kg = km.astype(np.int64)
+    b_batch = int(np.sum(4 * (Vg + 1) + 8 * Eg + 4 * Vg + 4 * Vg * (kg + 1)))
     out = {
         "workload": "%d R-MAT graphs (.57,.19,.19,.05), scale uniform over the integers 8..18, "
                     "edge factor uniform in [4,32], seed %d+g; concatenated int32 CSR in HBM"
@@ -267,6 +444,9 @@ def bench_batched(ctx, args):
         "plain_launches": sp["launches"], "plain_method": "peelone_dyn",
         "speedup_vs_plain": round(sp["total_ms"] / ms, 2),
         "autotuned": best, "levels_run": st["levels_run"], "kmax": st["kmax"],
+        "roofline": dict(_roof(b_batch, dev_med, traffic_from_profile("batch")),
+                         kernel="batch_kernel", kernel_ms=round(dev_med, 3),
+                         accounting="sum over graphs of the peel model with each graph's kmax"),
         "parity_graphs": "%d/%d" % (ok, G),
         "cpu_bz_s": round(cpu_s, 2), "cpu_bz_threads": threads,
         "gen_s": round(gen_s, 2),
@@ -295,7 +475,33 @@ def _pmc_per_kernel(db_path, counter):
     return {k: sum(v) / len(v) for k, v in out.items()}
 
 
-def summarize_pmc(paths, peel_stages=1):
+def profile_run(args):
+    """One method (or the batch) alone in the process, for rocprofv3 counter
+    passes: every dispatch of its kernel belongs to the workload named."""
+    import pico_amd as pa
+    ctx = pa.Context(0)
+    ctx.probe_copy(COPY_PROBE_BYTES, 2)
+    if args.profile == "batch":
+        db = ctx.gen_rmat_batch(args.batch_graphs, 8, 18, 4.0, 32.0, seed=args.seed)
+        for _ in range(2):
+            _, _, st = db.run(block=1024, per_cu=1, copy=False)
+        print(json.dumps({"profile": "batch", "device_ms": st["device_ms"],
+                          "launches": st["launches"]}), flush=True)
+        db.close()
+    else:
+        if args.graph == "chunglu":
+            dg = ctx.gen_chunglu(1 << 24, 2.1, 20.0, 4e6, seed=3, rowptr_bytes=8)
+        else:
+            dg = ctx.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+        for _ in range(2):
+            _, st = dg.decompose(args.profile, copy=False)
+        print(json.dumps({"profile": args.profile, "graph": args.graph,
+                          "device_ms": st["device_ms"], "stages": st["stages"]}), flush=True)
+        dg.close()
+    ctx.close()
+
+
+def summarize_pmc(paths, peel_stages=1, key=None, kernel=None):
     """profiles/pmc_traffic.json: HBM bytes per launch of the dominant
     kernels = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled so that the
     copy probe's counters read the bytes it is known to move.  One plain-peel
@@ -317,17 +523,28 @@ def summarize_pmc(paths, peel_stages=1):
                           "fetch_scale": sf, "write_scale": sw},
            "sources": ["FETCH_SIZE pass: " + os.path.basename(paths[0]),
                        "WRITE_SIZE pass: " + os.path.basename(paths[1])]}
-    for key, kname in (("peel", "peel_kernel"), ("hindex", "hindex_kernel"),
-                       ("batch", "batch_kernel")):
+    pairs = (((key, kernel),) if key else
+             (("peel", "peel_kernel"), ("hindex", "hindex_kernel"), ("batch", "batch_kernel")))
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if key:  # merge one entry into the existing summary
+        try:
+            with open(path) as fh:
+                merged = json.load(fh)
+        except (OSError, ValueError):
+            merged = {}
+        merged.setdefault("copy_probe", out["copy_probe"])
+        merged.setdefault("sources", [])
+        out = merged
+    for k, kname in pairs:
         f, w = pick(fetch, kname), pick(write, kname)
         if f is None or w is None:
             continue
         per = int(f * 1024.0 * sf + w * 1024.0 * sw)
-        n = peel_stages if key == "peel" else 1
-        out[key] = {"kernel": kname, "fetch_kib": f, "write_kib": w,
-                    "bytes_per_launch": per, "launches_per_step": n,
-                    "bytes_per_step": per * n}
-    with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as fh:
+        n = peel_stages if kname == "peel_kernel" else 1
+        out[k] = {"kernel": kname, "fetch_kib": f, "write_kib": w,
+                  "bytes_per_launch": per, "launches_per_step": n,
+                  "bytes_per_step": per * n, "fetch_scale": sf, "write_scale": sw}
+    with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
 
@@ -344,16 +561,27 @@ def main():
     import pico_amd as pa
 
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc, args.pmc_stages)
+        summarize_pmc(args.summarize_pmc, args.pmc_stages, args.pmc_key, args.pmc_kernel)
+        return
+    if args.profile:
+        profile_run(args)
         return
     if args.only == "peel":
-        args.no_batch = args.no_serve = True
-    if args.only in ("batch", "serve"):  # one section alone (profiling runs)
+        args.no_batch = args.no_serve = args.no_methods = args.no_chunglu = True
+        args.no_reader = True
+    if args.only == "methods":
+        args.no_batch = args.no_serve = args.no_chunglu = args.no_reader = True
+    if args.only == "reader":
+        args.no_batch = args.no_serve = args.no_chunglu = args.no_methods = True
+    if args.only in ("batch", "serve", "chunglu"):  # one section alone (profiling runs)
         if args.only == "serve":
             print(json.dumps({"serving": bench_serving(args)}), flush=True)
         else:
             c = pa.Context(0)
-            print(json.dumps({"batched": bench_batched(c, args)}), flush=True)
+            if args.only == "batch":
+                print(json.dumps({"batched": bench_batched(c, args)}), flush=True)
+            else:
+                print(json.dumps({"chunglu": bench_chunglu(c, args, check=False)}), flush=True)
             c.close()
         return
     ctx = pa.Context(0)
@@ -410,26 +638,49 @@ def main():
     h_ach_min = bh_used / (h_kernel_ms * 1e-3) / 1e9
 
     # ------------------------------------------------------- CPU baseline
+    # The reference has no CPU k-core; the baseline is the oracle's sequential
+    # restatement of the method the GPU runs here (peel-dyn.cc / peelone-dyn.cc:
+    # one full degree scan per level, then the level's queue), on one core,
+    # over the full graph.  The textbook Batagelj-Zaversnik peel is timed beside it.
     cpu = None
     parity = None
+    ref = None
     if not args.no_cpu:
         import oracle
         rp, col = g.rowptr, g.col
-        times = []
-        ref = None
+        times, bz_times = [], []
         for _ in range(max(1, args.cpu_runs)):
             t = time.perf_counter()
             ref = oracle.bz(rp, col)
+            bz_times.append(time.perf_counter() - t)
+        for _ in range(max(1, args.cpu_runs // 2)):
+            t = time.perf_counter()
+            pcore, _lv = oracle.bucket_peel(rp, col)
             times.append(time.perf_counter() - t)
         cpu_s = float(np.mean(times))
+        bz_s = float(np.mean(bz_times))
         cpu = {"value": round(m / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": "full R-MAT s22 graph, sequential Batagelj-Zaversnik peel "
-                         "(oracle/kc_oracle.c), mean of %d runs = %.3f s; host %s"
-                         % (len(times), cpu_s, cpu_model())}
-        parity = bool(np.array_equal(core, ref) and np.array_equal(hcore, ref))
+               "sample": "full R-MAT s%d graph (%d undirected edges), the sequential "
+                         "restatement of peel-dyn.cc's bucket peel (oracle/kc_oracle.c "
+                         "orc_bucket_peel), mean of %d runs = %.3f s" % (
+                             args.scale, m, len(times), cpu_s),
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "bz_gteps": round(m / bz_s / 1e9, 5), "bz_s": round(bz_s, 3),
+               "bz_runs": len(bz_times)}
+        parity = bool(np.array_equal(core, ref) and np.array_equal(hcore, ref) and
+                      np.array_equal(pcore, ref))
 
+    methods = None if args.no_methods else bench_methods(dg, V, nnz, m, ref,
+                                                         hst["changed_degree"])
+    reader = None if args.no_reader else bench_reader(ctx, g, args)
+    del g
+    chunglu = None if args.no_chunglu else bench_chunglu(ctx, args, check=not args.no_cpu)
     batched = None if args.no_batch else bench_batched(ctx, args)
-    serving = None if args.no_serve else bench_serving(args)
+    serving = sweep = None
+    if not args.no_serve:
+        serving = bench_serving(args)
+        sweep = [bench_serving(args, rate=float(r))
+                 for r in args.serve_sweep.split(",") if r.strip()]
 
     out = {
         "metric": "k-core GTEPS + % HBM roofline (R-MAT s22); batched vs plain graphs/s; p99",
@@ -477,8 +728,12 @@ def main():
                                 "atomics": hst["atomics"]}},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
+        "methods": methods,
+        "chunglu": chunglu,
         "batched": batched,
         "serving": serving,
+        "serving_sweep": sweep,
+        "reader": reader,
     }
     print(json.dumps(out), flush=True)
     dg.close()

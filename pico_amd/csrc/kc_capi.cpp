@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 #include <new>
 #include <string>
@@ -560,12 +561,47 @@ const uint32_t *kc_graph_core_device(const kc_graph *g) { return g ? g->result :
 // ----------------------------------------------------------- dispatching
 // Persistent engines wait at grid barriers for every workgroup of their
 // grid; two of them sharing a device from different streams could each hold
-// CUs the other's unscheduled workgroups need.  One lock per device keeps
-// them apart (everything else still overlaps).
-std::mutex &kci::coop_mutex(int device)
+// CUs the other's unscheduled workgroups need.  Every persistent launch
+// therefore reserves workgroup slots first, in arrival order, from a budget
+// of one 1024-thread workgroup per CU: launches of 1024-thread groups reserve
+// their grid width (their grids are then all resident together -- a CU holds
+// at least one such group), every other launch reserves the whole device.
+// Non-persistent kernels are not counted: they finish on their own.
+namespace {
+struct CoopState {
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t next_ticket = 0, serving = 0; // FIFO order of reservations
+    uint32_t used = 0;
+};
+
+CoopState &coop_state(int device)
 {
-    static std::mutex mu[64];
-    return mu[(unsigned)device & 63];
+    static CoopState s[64];
+    return s[(unsigned)device & 63];
+}
+} // namespace
+
+kci::CoopGuard::CoopGuard(int dev, uint32_t need, uint32_t capacity) : device(dev)
+{
+    if (need == 0 || need > capacity)
+        need = capacity;
+    CoopState &s = coop_state(dev);
+    std::unique_lock<std::mutex> lk(s.mu);
+    const uint64_t t = s.next_ticket++;
+    s.cv.wait(lk, [&] { return s.serving == t && s.used + need <= capacity; });
+    s.serving++;
+    s.used += need;
+    n = need;
+    s.cv.notify_all(); // the next reservation may fit beside this one
+}
+
+kci::CoopGuard::~CoopGuard()
+{
+    CoopState &s = coop_state(device);
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.used -= n;
+    s.cv.notify_all();
 }
 
 // Largest value of a[0..n): one atomic per workgroup (256 threads), not per
@@ -699,7 +735,11 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     };
     p.tail_live = tail_of(g->V);
     p.start_level = 0;
-    std::unique_lock<std::mutex> coop(coop_mutex(c->device));
+    // workgroups of the compacted stages' launches (see below)
+    const uint32_t stage_g =
+        std::max(1u, std::min(o.stage_grid ? o.stage_grid : maxgrid * 3 / 8, maxgrid));
+    CoopGuard coop(c->device, block == 1024 ? std::max(grid, p.tail_live ? stage_g : 0u) : 0u,
+                   c->num_cus);
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_peel(p, c->stream);
     if (e != hipSuccess)
@@ -801,8 +841,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
             // the compacted stages are latency-bound: fewer workgroups make
             // their barriers and shared-list polling cheaper (R-MAT s22 with
             // 1024-thread groups: 3/8 of the CUs is the measured optimum)
-            uint32_t sg = o.stage_grid ? o.stage_grid : std::max(1u, maxgrid * 3 / 8);
-            p.grid = std::max(1u, std::min(sg, maxgrid));
+            p.grid = stage_g;
         }
         if (p.trace) {
             uint64_t used = std::min<uint64_t>(acc_phases, KC_TRACE_CAP);
@@ -957,7 +996,7 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         p.trace = (uint64_t *)g->trace.p;
         p.trace_cap = KC_TRACE_CAP;
     }
-    std::unique_lock<std::mutex> coop(coop_mutex(c->device));
+    CoopGuard coop(c->device, 0u, c->num_cus); // the whole device
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
     hipError_t e = kc::launch_hindex(p, c->stream);
     if (e != hipSuccess)

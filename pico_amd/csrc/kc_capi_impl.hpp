@@ -99,8 +99,18 @@ static inline kc_opts with_tuned(const kc_graph *g, kc_method method, kc_opts o)
 }
 
 namespace kci {
-// one lock per device: persistent grid-barrier launches never share the CUs
-std::mutex &coop_mutex(int device);
+// Workgroup slots reserved by a persistent (grid-barrier) launch on a device
+// for its lifetime, in arrival order: `need` 1024-thread workgroups out of a
+// budget of `capacity` (one per CU); need 0 (or above the budget) reserves
+// the whole device.
+struct CoopGuard {
+    int device;
+    uint32_t n = 0;
+    CoopGuard(int device, uint32_t need, uint32_t capacity);
+    ~CoopGuard();
+    CoopGuard(const CoopGuard &) = delete;
+    CoopGuard &operator=(const CoopGuard &) = delete;
+};
 // default persistent grid width for a graph of nnz adjacency entries
 uint32_t size_grid(uint64_t nnz);
 // locate the static scratch array of kc_decompose_noalloc on the current

@@ -156,7 +156,7 @@ extern "C" kc_status kc_decompose_noalloc(kc_ctx *ctx, uint32_t V, uint64_t nnz,
     uint64_t sv[kc::KC_ST_COUNT];
     float ms = 0;
     {
-        std::unique_lock<std::mutex> coop(coop_mutex(ctx->device));
+        CoopGuard coop(ctx->device, block == 1024 ? grid : 0u, ctx->num_cus);
         KC_HIP(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
         hipError_t e = kc::launch_peel(p, ctx->stream);
         if (e != hipSuccess)

@@ -228,13 +228,19 @@ void run_job(kc_server *s, kc_ctx *ctx, std::vector<Req *> &job, std::vector<kc_
     kc_completion c = make_completion(q0, t_start);
     c.fused = 1;
     kc_status st = KC_OK;
+    // A large decomposition takes a share of the CUs (its grid and its
+    // compacted stages' grid), so that as many as there are workers run side
+    // by side instead of one after another; cores do not depend on the grid.
+    kc_opts share{};
+    share.grid = std::max(16u, ctx->num_cus / std::max(1u, s->o.slots));
+    share.stage_grid = std::min(share.grid, std::max(1u, ctx->num_cus * 3 / 8));
     if (q0->r.kind == KC_REQ_DECOMPOSE_RMAT) {
         kc_graph *g = nullptr;
         st = kc_graph_gen_rmat(ctx, q0->r.scale, rmat_m_gen(q0->r), RMAT_A, RMAT_B, RMAT_C,
                                q0->r.seed, 4, &g);
         kc_stats ks{};
         if (st == KC_OK)
-            st = kc_decompose(g, KC_PEELONE_DYN, nullptr, q0->r.core_out, &ks);
+            st = kc_decompose(g, KC_PEELONE_DYN, &share, q0->r.core_out, &ks);
         if (st == KC_OK)
             st = checksum_of(g->result, g->V, ctx->stream, c.checksum);
         c.V = g ? g->V : 0;
@@ -249,7 +255,7 @@ void run_job(kc_server *s, kc_ctx *ctx, std::vector<Req *> &job, std::vector<kc_
             c.applied = us.applied;
         } else {
             kc_stats ks{};
-            st = kc_decompose(g, KC_PEELONE_DYN, nullptr, q0->r.core_out, &ks);
+            st = kc_decompose(g, KC_PEELONE_DYN, &share, q0->r.core_out, &ks);
         }
         if (st == KC_OK)
             st = checksum_of(g->result, g->V, q0->gctx->stream, c.checksum);
