@@ -73,6 +73,13 @@ def parse():
                          "write (merged into the file)")
     ap.add_argument("--pmc-kernel", default=None,
                     help="with --summarize-pmc and --pmc-key: kernel name substring")
+    ap.add_argument("--summarize-trace", metavar="KT_DB", default=None,
+                    help="print the kernel table of a rocprofv3 --kernel-trace database")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="X", default=None,
+                    help="DB KERNEL COUNTER...: per-dispatch means of counters for one kernel")
+    ap.add_argument("--pmc-runs", type=int, default=2,
+                    help="with --summarize-pmc and --pmc-key: decompositions in the profiled "
+                         "--profile run (it runs two)")
     ap.add_argument("--pmc-stages", type=int, default=1,
                     help="with --summarize-pmc: peel_kernel launches per plain-peel step "
                          "(launches_per_step of the profiled run)")
@@ -458,21 +465,57 @@ def bench_batched(ctx, args):
 COPY_PROBE_BYTES = 1 << 30
 
 
-def _pmc_per_kernel(db_path, counter):
-    """Mean value of `counter` per dispatch, by kernel name, from a rocprofv3
+def _pmc_dispatches(db_path, counter):
+    """Values of `counter` per dispatch, by kernel name, from a rocprofv3
     rocpd database."""
     import sqlite3
     db = sqlite3.connect(db_path)
     cur = db.cursor()
-    view, name_col, disp_col, cnt_col, val_col = ("pmc_events", "name", "dispatch_id",
-                                                   "counter_name", "counter_value")
-    rows = cur.execute("select %s, %s, sum(%s) from %s where %s = ? group by %s, %s"
-                       % (name_col, disp_col, val_col, view, cnt_col, name_col, disp_col),
+    rows = cur.execute("select name, dispatch_id, sum(counter_value) from pmc_events "
+                       "where counter_name = ? group by name, dispatch_id",
                        (counter,)).fetchall()
     out = {}
     for name, _, v in rows:
         out.setdefault(name, []).append(float(v))
-    return {k: sum(v) / len(v) for k, v in out.items()}
+    return out
+
+
+def _pmc_per_kernel(db_path, counter):
+    """Mean value of `counter` per dispatch, by kernel name."""
+    return {k: sum(v) / len(v) for k, v in _pmc_dispatches(db_path, counter).items()}
+
+
+def summarize_trace(db_path, title=None):
+    """Markdown table of a rocprofv3 --kernel-trace database: calls and
+    mean / min / max / total duration per kernel name."""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    rows = cur.execute("select name, count(*), avg(end - start), min(end - start), "
+                       "max(end - start), sum(end - start) from kernels group by name "
+                       "order by sum(end - start) desc").fetchall()
+    tot = sum(r[5] for r in rows) or 1
+    lines = ["| kernel | calls | avg (us) | min (us) | max (us) | total % |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for name, n, avg, mn, mx, sm in rows:
+        nm = name if len(name) <= 90 else name[:87] + "..."
+        lines.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f |" % (
+            nm.replace("|", "\\|"), n, avg / 1e3, mn / 1e3, mx / 1e3, 100.0 * sm / tot))
+    if title:
+        print("### " + title)
+    print("\n".join(lines))
+
+
+def summarize_counters(db_path, counters, kernel):
+    """Mean per dispatch and total of each counter for the kernels whose name
+    contains `kernel` (the counting variant of peel_kernel excluded)."""
+    out = {}
+    for c in counters:
+        d = _pmc_dispatches(db_path, c)
+        vals = [v for k, vs in d.items() if kernel in k and "true>" not in k for v in vs]
+        out[c] = {"dispatches": len(vals), "mean": (sum(vals) / len(vals)) if vals else None,
+                  "total": sum(vals)}
+    print(json.dumps(out, indent=1))
 
 
 def profile_run(args):
@@ -501,7 +544,7 @@ def profile_run(args):
     ctx.close()
 
 
-def summarize_pmc(paths, peel_stages=1, key=None, kernel=None):
+def summarize_pmc(paths, peel_stages=1, key=None, kernel=None, runs=2):
     """profiles/pmc_traffic.json: HBM bytes per launch of the dominant
     kernels = FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled so that the
     copy probe's counters read the bytes it is known to move.  One plain-peel
@@ -535,12 +578,19 @@ def summarize_pmc(paths, peel_stages=1, key=None, kernel=None):
         merged.setdefault("copy_probe", out["copy_probe"])
         merged.setdefault("sources", [])
         out = merged
+    fd = _pmc_dispatches(paths[0], "FETCH_SIZE")
     for k, kname in pairs:
         f, w = pick(fetch, kname), pick(write, kname)
         if f is None or w is None:
             continue
         per = int(f * 1024.0 * sf + w * 1024.0 * sw)
-        n = peel_stages if kname == "peel_kernel" else 1
+        if key:
+            # a --profile run: its decompositions each launch the kernel the
+            # same number of times (one per compacted stage)
+            nd = sum(len(v) for kk, v in fd.items() if kname in kk and "true>" not in kk)
+            n = max(1, round(nd / max(1, runs)))
+        else:
+            n = peel_stages if kname == "peel_kernel" else 1
         out[k] = {"kernel": kname, "fetch_kib": f, "write_kib": w,
                   "bytes_per_launch": per, "launches_per_step": n,
                   "bytes_per_step": per * n, "fetch_scale": sf, "write_scale": sw}
@@ -560,8 +610,16 @@ def main():
 
     import pico_amd as pa
 
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters[0], args.summarize_counters[2:],
+                           args.summarize_counters[1])
+        return
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc, args.pmc_stages, args.pmc_key, args.pmc_kernel)
+        summarize_pmc(args.summarize_pmc, args.pmc_stages, args.pmc_key, args.pmc_kernel,
+                      args.pmc_runs)
         return
     if args.profile:
         profile_run(args)
