@@ -331,7 +331,9 @@ kc_status kc_hg_vgraph(const kc_host_graph *g, uint32_t part_size, uint32_t *nod
         if (edge_list) {
             uint64_t b = g->rowptr[v] + v;
             edge_list[b] = (uint32_t)d;
-            std::memcpy(edge_list + b + 1, g->col.data() + g->rowptr[v], d * sizeof(uint32_t));
+            if (d)
+                std::memcpy(edge_list + b + 1, g->col.data() + g->rowptr[v],
+                            d * sizeof(uint32_t));
         }
         if (parts)
             for (uint64_t j = 0; j < np; j++) {
@@ -1154,7 +1156,8 @@ extern "C" kc_status kc_vgraph_decompose(uint32_t num_nodes, uint64_t num_edges2
         if (pos + d > num_edges2)
             return fail(KC_EINVAL, "degree of vertex %u overruns the edge list", v);
         rp[v] = pos;
-        std::memcpy(col.data() + pos, edge_list + np + 1, (size_t)d * 4);
+        if (d)
+            std::memcpy(col.data() + pos, edge_list + np + 1, (size_t)d * 4);
         pos += d;
     }
     rp[num_nodes] = pos;

@@ -77,7 +77,9 @@ int csr_from_pairs(uint32_t V, uint64_t n, const uint32_t *src, const uint32_t *
     g.col.resize(g.rowptr[V]);
     parallel_for(V, 4096, [&](uint64_t b, uint64_t e) {
         for (uint64_t v = b; v < e; v++)
-            std::memcpy(g.col.data() + g.rowptr[v], tmp.data() + off[v], ud[v] * sizeof(uint32_t));
+            if (ud[v])
+                std::memcpy(g.col.data() + g.rowptr[v], tmp.data() + off[v],
+                            ud[v] * sizeof(uint32_t));
     });
     g.multigraph = false;
     return 0;
