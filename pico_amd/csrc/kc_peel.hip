@@ -26,7 +26,9 @@ namespace kc {
 
 enum { MODE_BUCKET = 0, MODE_PEELONE = 1, MODE_GPP = 2 };
 
-template <typename R> struct PeelArgs {
+// M: the peel mode (MODE_BUCKET, MODE_PEELONE, MODE_GPP), a template parameter so
+// that each instance carries only its own paths (and registers)
+template <typename R, int M> struct PeelArgs {
     const R *rowptr;
     const uint32_t *col;
     uint32_t V;
@@ -110,8 +112,8 @@ __device__ __forceinline__ uint32_t find_item64(const uint64_t *pre, uint32_t n,
 
 // Wave-aggregated append of the lanes' `want` items to the shared list of
 // this phase.
-template <typename R>
-__device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase, bool want,
+template <typename R, int M>
+__device__ __forceinline__ void push_shared(const PeelArgs<R, M> &a, uint32_t phase, bool want,
                                             uint64_t item)
 {
     uint64_t m = ballot(want);
@@ -124,7 +126,7 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
     uint32_t base = 0;
     if (lane == leader) {
         uint32_t k = (uint32_t)__popcll(m);
-        if (a.mode == MODE_GPP)
+        if (M == MODE_GPP)
             base = atomicAdd(&slot->qtail, k);
         else // tail and pending work move together
             base = (uint32_t)(atomicAdd((unsigned long long *)&slot->qword,
@@ -150,8 +152,8 @@ __device__ __forceinline__ void push_shared(const PeelArgs<R> &a, uint32_t phase
 // shared list whole and are claimed when the next iteration retires them.
 constexpr uint32_t KC_WHOLE = 0xffffffffu; // shared item: a whole vertex, split by its taker
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, const PeelArgs<R> &a,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, const PeelArgs<R, M> &a,
                                             uint32_t phase, uint32_t level, bool claim,
                                             uint32_t u)
 {
@@ -159,7 +161,7 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
     if (!cm)
         return;
     uint32_t lane = lane_id();
-    if (a.mode == MODE_GPP) {
+    if (M == MODE_GPP) {
         push_shared(a, phase, claim, (uint64_t)u);
         return;
     }
@@ -185,8 +187,8 @@ __device__ __forceinline__ void push_claims(PeelLds<BLOCK, LQ> &L, int buf, cons
 }
 
 // Push chunks [1, nch) of hub v (chunk 0 stays with the caller).
-template <typename R>
-__device__ __forceinline__ void push_hub_chunks(const PeelArgs<R> &a, uint32_t phase, uint32_t v,
+template <typename R, int M>
+__device__ __forceinline__ void push_hub_chunks(const PeelArgs<R, M> &a, uint32_t phase, uint32_t v,
                                                 uint32_t nch)
 {
     if (nch <= 1)
@@ -259,8 +261,8 @@ __device__ __forceinline__ bool agg_add(PeelLds<BLOCK, LQ> &L, uint32_t u)
 // first subtraction to saturate sees every earlier one already settled, so
 // the effective amounts sum to min(total, degree - level) in any
 // interleaving, as the one-at-a-time rule does.
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                           uint32_t phase, int pushbuf)
 {
     uint32_t na = 0;
@@ -286,14 +288,14 @@ __device__ void flush_agg(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                 claim = (pv & bit) == 0;
             }
         }
-        push_claims<BLOCK, LQ, R, CNT>(L, pushbuf, a, phase, level, claim, u);
+        push_claims<BLOCK, LQ, R, CNT, M>(L, pushbuf, a, phase, level, claim, u);
     }
     if constexpr (CNT)
         count_atomics(L, na);
 }
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                            uint32_t phase, uint32_t m, int pushbuf)
 {
     const uint64_t total = L.pre[m];
@@ -321,17 +323,17 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
             pre[k] = 0xffffffffull;
-            if (val[k] && (a.mode == MODE_GPP || pc))
+            if (val[k] && (M == MODE_GPP || pc))
                 // GPP needs the exact claim word; the clamp pre-check may
                 // read a stale degree (stale values only ever lie above)
-                pre[k] = a.mode == MODE_GPP ? ld_relaxed64(&a.done[u[k] >> 6])
+                pre[k] = M == MODE_GPP ? ld_relaxed64(&a.done[u[k] >> 6])
                                             : (uint64_t)a.deg[u[k]];
         }
         uint32_t old[WALK_U];
         bool hit[WALK_U];
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
-            if (a.mode == MODE_GPP) {
+            if (M == MODE_GPP) {
                 hit[k] = val[k] && !((pre[k] >> (u[k] & 63)) & 1ull);
             } else {
                 hit[k] = val[k] && (uint32_t)pre[k] > level;
@@ -349,11 +351,11 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
             claim[k] = false;
             if (!hit[k])
                 continue;
-            na += 1 + (a.mode != MODE_GPP && old[k] <= level) +
-                  (a.mode != MODE_GPP && old[k] == level + 1);
+            na += 1 + (M != MODE_GPP && old[k] <= level) +
+                  (M != MODE_GPP && old[k] == level + 1);
             if (old[k] > level + 1 && old[k] - 1 < nmin)
                 nmin = old[k] - 1;
-            if (a.mode == MODE_GPP) {
+            if (M == MODE_GPP) {
                 claim[k] = old[k] == level + 1;
             } else if (old[k] == level + 1) {
                 // bucket peels and PeelOne alike: a vertex lowered onto the
@@ -370,7 +372,7 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
         }
 #pragma unroll
         for (int k = 0; k < WALK_U; k++)
-            push_claims<BLOCK, LQ, R, CNT>(L, pushbuf, a, phase, level, claim[k], u[k]);
+            push_claims<BLOCK, LQ, R, CNT, M>(L, pushbuf, a, phase, level, claim[k], u[k]);
         nmin = wave_min_u32(nmin);
         if (lane_id() == 0 && nmin != 0xffffffffu)
             atomicMin(&L.minv, nmin);
@@ -378,8 +380,8 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
     if constexpr (CNT)
         count_atomics(L, na);
     __syncthreads();
-    if (a.mode != MODE_GPP) {
-        flush_agg<BLOCK, LQ, R, CNT>(L, a, level, phase, pushbuf);
+    if (M != MODE_GPP) {
+        flush_agg<BLOCK, LQ, R, CNT, M>(L, a, level, phase, pushbuf);
         __syncthreads();
     }
 }
@@ -419,8 +421,8 @@ __device__ __forceinline__ void stage_prefix(PeelLds<BLOCK, LQ> &L, uint32_t m, 
 
 // Process the local frontier until it is empty.  Items claimed while
 // processing buffer b are pushed to buffer b^1.
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level, uint32_t phase,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level, uint32_t phase,
                       uint64_t &edges)
 {
     int cur = 0;
@@ -463,7 +465,7 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
                 stage_prefix<BLOCK, LQ>(L, keep, threadIdx.x < keep ? len : 0u);
             }
             edges += threadIdx.x < keep ? len : 0u;
-            walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, keep, cur ^ 1);
+            walk_items<BLOCK, LQ, R, CNT, M>(L, a, level, phase, keep, cur ^ 1);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -484,8 +486,8 @@ __device__ void drain(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t leve
 // drops a chunk once its claim word is full.  Every wave keeps four chunks'
 // loads in flight.  Bucket/PeelOne claims go to lbuf[0]; GPP retirees are
 // written out and listed for next phase's scatter.
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a,
                                            uint32_t level, uint32_t phase, uint32_t c,
                                            uint64_t dw, uint32_t d, bool live, uint32_t &lmin,
                                            uint32_t &nret)
@@ -511,18 +513,18 @@ __device__ __forceinline__ void scan_chunk(PeelLds<BLOCK, LQ> &L, const PeelArgs
         prev = __shfl(prev, leader, KC_WAVE);
         claim = ((cm & ~(uint64_t)prev) >> lane) & 1ull;
     }
-    if (a.mode == MODE_GPP) {
+    if (M == MODE_GPP) {
         if (claim)
             a.out[v] = level;
         nret += claim ? 1u : 0u;
         push_shared(a, phase, claim, (uint64_t)v);
     } else {
-        push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v);
+        push_claims<BLOCK, LQ, R, CNT, M>(L, 0, a, phase, level, claim, v);
     }
 }
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                            uint32_t phase)
 {
     constexpr uint32_t NW = BLOCK / KC_WAVE;
@@ -568,7 +570,7 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
                     L.chunks[i0 + NW * k] = KC_DEAD; // every vertex claimed
                 continue;
             }
-            scan_chunk<BLOCK, LQ, R, CNT>(L, a, level, phase, cc[k], dw[k], d[k], lv[k], lmin, nret);
+            scan_chunk<BLOCK, LQ, R, CNT, M>(L, a, level, phase, cc[k], dw[k], d[k], lv[k], lmin, nret);
         }
     }
     lmin = wave_min_u32(lmin);
@@ -603,8 +605,8 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t
 // vertices of its chunks into its own region of a list (one reservation of
 // the region, then each thread writes its chunk's vertices), and the scans of
 // later levels read and compact that region instead of whole chunks.
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a)
 {
     const bool use_list = L.use_list != 0;
     const uint32_t n = use_list ? L.nlive : L.ncw;
@@ -645,8 +647,8 @@ __device__ void build_live_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a)
     __syncthreads();
 }
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                           uint32_t phase)
 {
     constexpr uint32_t U = 4;
@@ -693,7 +695,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                                        (unsigned long long)bit);
                 claim = (pv & bit) == 0;
             }
-            if (a.mode == MODE_GPP) { // retire and list for the scatter, as scan_chunk
+            if (M == MODE_GPP) { // retire and list for the scatter, as scan_chunk
                 if (claim)
                     a.out[v[k]] = level;
                 uint64_t rm = ballot(claim);
@@ -701,7 +703,7 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
                     atomicAdd(&L.claims, (uint32_t)__popcll(rm));
                 push_shared(a, phase, claim, (uint64_t)v[k]);
             } else {
-                push_claims<BLOCK, LQ, R, CNT>(L, 0, a, phase, level, claim, v[k]);
+                push_claims<BLOCK, LQ, R, CNT, M>(L, 0, a, phase, level, claim, v[k]);
             }
             uint64_t km = ballot(keep);
             if (km) {
@@ -736,8 +738,8 @@ __device__ void scan_list(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t 
 //  kind RETIRE : items are GPP crossers, retired here, then scattered
 enum { TAKE_CHUNKS = 0, TAKE_WHOLE = 1, TAKE_RETIRE = 2 };
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                             uint32_t phase, uint32_t n_items, int kind, uint64_t &edges)
 {
     PhaseSlot *prev = &slot_of(a.gs, phase + 2)[0];
@@ -787,9 +789,9 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
         }
         edges += len;
         stage_prefix<BLOCK, LQ>(L, m, len);
-        walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, m, 0);
-        if (a.mode != MODE_GPP)
-            drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
+        walk_items<BLOCK, LQ, R, CNT, M>(L, a, level, phase, m, 0);
+        if (M != MODE_GPP)
+            drain<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
     }
 }
 
@@ -808,8 +810,8 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 // and every reserved slot at or past the tail is dropped.  Ready items are always processed
 // before waiting on the rest of a reservation, so no workgroup sits on work
 // another one waits for.
-template <int BLOCK, int LQ, typename R, bool CNT>
-__device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_t level,
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                             uint32_t phase, uint64_t &edges)
 {
     PhaseSlot *slot = &slot_of(a.gs, phase)[0];
@@ -928,8 +930,8 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
                 len = threadIdx.x < k ? L.hlen[threadIdx.x] : 0u;
                 edges += len;
                 stage_prefix<BLOCK, LQ>(L, k, len);
-                walk_items<BLOCK, LQ, R, CNT>(L, a, level, phase, k, 0);
-                drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
+                walk_items<BLOCK, LQ, R, CNT, M>(L, a, level, phase, k, 0);
+                drain<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
                 vm_drain(); // the pushes these items caused come first
                 __syncthreads();
                 if (threadIdx.x == 0)
@@ -982,8 +984,8 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R> &a, uint32_
 // phase kinds of the persistent loop
 enum { PH_SCAN = 0, PH_SHARED = 1, PH_GPP_SCATTER = 2, PH_GPP_CROSS = 3, PH_DONE = 4 };
 
-template <int BLOCK, int LQ, typename R, bool CNT>
-__global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
+template <int BLOCK, int LQ, typename R, bool CNT, int M>
+__global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R, M> a)
 {
     __shared__ PeelLds<BLOCK, LQ> L;
     GridSync *gs = a.gs;
@@ -1002,7 +1004,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
     }
 
     uint32_t gen = 0;
-    uint32_t level = (a.mode == MODE_GPP) ? 1u : a.start_level;
+    uint32_t level = (M == MODE_GPP) ? 1u : a.start_level;
     uint32_t tail_level = 0xffffffffu; // level at which the kernel stopped early
     uint64_t total = 0; // vertices claimed so far (identical in all WGs)
     uint64_t lvl_claims = 0;
@@ -1038,7 +1040,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         // Few vertices left: stop before this level's scan so the caller can
         // compact the live subgraph and continue on it (the state is that of
         // a completed level; every workgroup takes the same decision).
-        if (a.mode != MODE_GPP && kind == PH_SCAN && a.tail_live && total < a.V &&
+        if (M != MODE_GPP && kind == PH_SCAN && a.tail_live && total < a.V &&
             (uint64_t)a.V - total <= a.tail_live) {
             tail_level = level;
             break;
@@ -1053,8 +1055,11 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         }
         __syncthreads();
         const uint32_t phase = gen;
-        uint64_t *tr = (a.trace && phase < a.trace_cap) ? a.trace + (uint64_t)phase * KC_TRACE_WORDS
-                                                        : nullptr;
+        // the per-phase trace is recorded by the diagnostic instance (CNT),
+        // which also counts atomics; the fast instance carries neither
+        uint64_t *tr = (CNT && a.trace && phase < a.trace_cap)
+                           ? a.trace + (uint64_t)phase * KC_TRACE_WORDS
+                           : nullptr;
         const uint64_t edges_before = edges;
         if (tr && threadIdx.x == 0) {
             uint64_t t = now_ticks();
@@ -1063,25 +1068,31 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             atomicMax((unsigned long long *)&tr[14], (unsigned long long)t);
         }
         if (kind == PH_SCAN) {
-            if (a.mode != MODE_PEELONE && !L.lmode && a.live_thresh &&
-                (uint64_t)a.V - total <= a.live_thresh)
-                build_live_list<BLOCK, LQ, R, CNT>(L, a);
-            if (L.lmode)
-                scan_list<BLOCK, LQ, R, CNT>(L, a, level, phase);
+            // GPP's levels start with a full scan: once few vertices are
+            // left it reads a compact live-vertex list.  The cascading modes
+            // scan their live-chunk lists (and run on compacted subgraphs):
+            // without the list path their instances fit in the registers of
+            // a 1,024-thread group.
+            if constexpr (M == MODE_GPP) {
+                if (!L.lmode && a.live_thresh && (uint64_t)a.V - total <= a.live_thresh)
+                    build_live_list<BLOCK, LQ, R, CNT, M>(L, a);
+            }
+            if (M == MODE_GPP && L.lmode)
+                scan_list<BLOCK, LQ, R, CNT, M>(L, a, level, phase);
             else
-                scan_slice<BLOCK, LQ, R, CNT>(L, a, level, phase);
+                scan_slice<BLOCK, LQ, R, CNT, M>(L, a, level, phase);
             if (tr && threadIdx.x == 0)
                 atomicMax((unsigned long long *)&tr[12], (unsigned long long)now_ticks());
-            if (a.mode != MODE_GPP) {
-                drain<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
+            if (M != MODE_GPP) {
+                drain<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
                 if (tr && threadIdx.x == 0)
                     atomicMax((unsigned long long *)&tr[13], (unsigned long long)now_ticks());
-                help_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, edges);
+                help_shared<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
             }
         } else if (kind == PH_SHARED) {
-            take_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
+            take_shared<BLOCK, LQ, R, CNT, M>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
         } else {
-            take_shared<BLOCK, LQ, R, CNT>(L, a, level, phase, n_prev,
+            take_shared<BLOCK, LQ, R, CNT, M>(L, a, level, phase, n_prev,
                                       kind == PH_GPP_CROSS ? TAKE_RETIRE : TAKE_WHOLE, edges);
         }
         __syncthreads();
@@ -1115,7 +1126,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             break;
         PhaseSlot *s = &slot_of(gs, phase)[0];
         uint64_t claims = ld_relaxed64(&s->sum0);
-        uint32_t n_next = a.mode == MODE_GPP ? ld_relaxed(&s->qtail)
+        uint32_t n_next = M == MODE_GPP ? ld_relaxed(&s->qtail)
                                              : (uint32_t)(ld_relaxed64(&s->qword) >> 32);
         uint32_t mn = ld_relaxed(&s->min0);
         uint32_t live = ld_relaxed(&s->flag);
@@ -1127,7 +1138,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             tr[7] = (uint64_t)kind;
         }
 
-        if (a.mode == MODE_GPP) {
+        if (M == MODE_GPP) {
             // iteration accounting of gppeel.cu:186-228
             if (kind == PH_SCAN) {
                 iters++;
@@ -1174,7 +1185,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
             kmax = level;
         if (total >= a.V)
             break;
-        if (a.mode == MODE_PEELONE) {
+        if (M == MODE_PEELONE) {
             // peelone.cu's host loop (:154-186) sweeps a level again after a
             // sweep that fired and moves on after one that did not.  Every
             // vertex lowered onto the level fired in this sweep, so that
@@ -1202,7 +1213,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R> a)
         kind = PH_SCAN;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.stats[KC_ST_ITERS] = iters + (a.mode == MODE_PEELONE ? 1 : 0);
+        a.stats[KC_ST_ITERS] = iters + (M == MODE_PEELONE ? 1 : 0);
         a.stats[KC_ST_KMAX] = kmax;
         a.stats[KC_ST_PHASES] = gen;
         a.stats[KC_ST_LEVELS_RUN] = levels_run;
@@ -1257,10 +1268,10 @@ __global__ void gpp_fix_isolated(const R *rowptr, uint32_t V, uint32_t *out)
 }
 
 // ------------------------------------------------------------- launchers ---
-template <int BLOCK, int LQ, typename R>
-static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
+template <int BLOCK, int LQ, typename R, int M>
+static hipError_t launch_peel_m(const PeelLaunch &p, hipStream_t st)
 {
-    PeelArgs<R> a;
+    PeelArgs<R, M> a;
     a.rowptr = (const R *)p.rowptr;
     a.col = p.col;
     a.V = p.V;
@@ -1271,52 +1282,79 @@ static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
     a.list0 = p.list0;
     a.list1 = p.list1;
     a.list_cap = p.list_cap;
-    a.mode = p.mode;
+    a.mode = (uint32_t)M;
     a.out = p.out;
-    a.live = p.mode == MODE_GPP ? p.live : p.out;
-    a.live_thresh = p.mode != MODE_PEELONE && a.live ? p.live_thresh : 0u;
+    a.live = M == MODE_GPP ? p.live : p.out;
+    a.live_thresh = M != MODE_PEELONE && a.live ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
-    a.tail_live = p.mode != MODE_GPP ? p.tail_live : 0u;
+    a.tail_live = M != MODE_GPP ? p.tail_live : 0u;
     a.abort_flag = p.abort_flag;
-    a.start_level = p.mode != MODE_GPP ? p.start_level : 0u;
+    a.start_level = M != MODE_GPP ? p.start_level : 0u;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
     a.trace_cap = p.trace_cap;
     uint32_t ndone = (p.V + 63) / 64;
     hipLaunchKernelGGL((peel_init_kernel<R>), dim3(1024), dim3(256), 0, st, a.rowptr, p.V, p.deg,
-                       p.done, ndone, p.mode == MODE_GPP ? p.out : nullptr, p.gs, p.grid,
+                       p.done, ndone, M == MODE_GPP ? p.out : nullptr, p.gs, p.grid,
                        p.timeout_ticks, p.stats, p.list0, p.list1,
-                       p.mode == MODE_GPP ? 0u : p.list_cap);
+                       M == MODE_GPP ? 0u : p.list_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     void *args[] = {&a};
-    const void *fn = p.count_atomics ? (const void *)peel_kernel<BLOCK, LQ, R, true>
-                                     : (const void *)peel_kernel<BLOCK, LQ, R, false>;
+    const bool diag = p.count_atomics || p.trace;
+    const void *fn = diag ? (const void *)peel_kernel<BLOCK, LQ, R, true, M>
+                          : (const void *)peel_kernel<BLOCK, LQ, R, false, M>;
     e = launch_persistent(fn, dim3(p.grid), dim3(BLOCK), args, st);
     if (e != hipSuccess)
         return e;
-    if (p.mode == MODE_GPP && !p.gpp_compat && p.V)
+    if (M == MODE_GPP && !p.gpp_compat && p.V)
         hipLaunchKernelGGL((gpp_fix_isolated<R>), dim3((p.V + 255) / 256), dim3(256), 0, st,
                            a.rowptr, p.V, p.out);
     return hipGetLastError();
 }
 
-template <int BLOCK, int LQ, typename R, bool CNT> static int occupancy_k()
+template <int BLOCK, int LQ, typename R>
+static hipError_t launch_peel_t(const PeelLaunch &p, hipStream_t st)
+{
+    switch (p.mode) {
+    case MODE_BUCKET: return launch_peel_m<BLOCK, LQ, R, MODE_BUCKET>(p, st);
+    case MODE_PEELONE: return launch_peel_m<BLOCK, LQ, R, MODE_PEELONE>(p, st);
+    case MODE_GPP: return launch_peel_m<BLOCK, LQ, R, MODE_GPP>(p, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BLOCK, int LQ, typename R, bool CNT, int M> static int occupancy_k()
 {
     int nb = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, peel_kernel<BLOCK, LQ, R, CNT>, BLOCK, 0);
+        &nb, peel_kernel<BLOCK, LQ, R, CNT, M>, BLOCK, 0);
     return e == hipSuccess ? nb : 0;
 }
 
-// The grid must be co-resident with either variant (counting or not).
+template <int BLOCK, int LQ, typename R, int M> static int occupancy_m()
+{
+    int a = occupancy_k<BLOCK, LQ, R, false, M>(), b = occupancy_k<BLOCK, LQ, R, true, M>();
+    return a < b ? a : b;
+}
+
+// The grid must be co-resident with every instance (each mode, counting or not).
 template <int BLOCK, int LQ> static int occupancy_t(bool wide)
 {
-    int a = wide ? occupancy_k<BLOCK, LQ, uint64_t, false>() : occupancy_k<BLOCK, LQ, uint32_t, false>();
-    int b = wide ? occupancy_k<BLOCK, LQ, uint64_t, true>() : occupancy_k<BLOCK, LQ, uint32_t, true>();
-    return a < b ? a : b;
+    int o[3];
+    if (wide) {
+        o[0] = occupancy_m<BLOCK, LQ, uint64_t, MODE_BUCKET>();
+        o[1] = occupancy_m<BLOCK, LQ, uint64_t, MODE_PEELONE>();
+        o[2] = occupancy_m<BLOCK, LQ, uint64_t, MODE_GPP>();
+    } else {
+        o[0] = occupancy_m<BLOCK, LQ, uint32_t, MODE_BUCKET>();
+        o[1] = occupancy_m<BLOCK, LQ, uint32_t, MODE_PEELONE>();
+        o[2] = occupancy_m<BLOCK, LQ, uint32_t, MODE_GPP>();
+    }
+    int m = o[0] < o[1] ? o[0] : o[1];
+    return m < o[2] ? m : o[2];
 }
 
 int peel_occupancy(uint32_t block, bool wide)
