@@ -86,7 +86,7 @@ typedef struct kc_opts {
     uint32_t flags;      /* KC_OPT_* */
     uint32_t agg_div;    /* bucket peels: read each target's degree before decrementing and
                             sum hub decrements in LDS once at most V/agg_div vertices are
-                            live (0: 64; 0xffffffff: from the start) */
+                            live (0: 128; 0xffffffff: from the start) */
     uint32_t compact_div; /* bucket peels and PeelOne: once at most V/compact_div vertices are live (and
                              at least 4,096), continue on a compacted copy of the live
                              subgraph, again at each further 1/compact_div; 0: default (8),
@@ -238,11 +238,11 @@ KC_API kc_status kc_graph_histogram(const kc_graph *g, uint32_t *out, uint64_t n
  * KC_PEELONE_DYN, one persistent launch, no residual compaction) on a CSR
  * the caller holds in device memory on ctx's device -- rowptr u32[V+1],
  * col u32[nnz] -- writing the V core numbers to the caller's device buffer
- * core.  Every work buffer (claim bitmap, work lists, live-vertex list,
- * barrier and statistics block) lives in one static device array of
+ * core.  Every work buffer (claim bitmap, work lists, barrier and
+ * statistics block) lives in one static device array of
  * KC_NOALLOC_SCRATCH_BYTES on each GPU, held until the process ends; the
  * call makes no device allocation.  A graph whose work buffers need more
- * (about 20 bytes per vertex plus 16 per 1,024 adjacency entries: R-MAT
+ * (about 16 bytes per vertex plus 16 per 1,024 adjacency entries: R-MAT
  * scale 22 fits) is refused with KC_ENOMEM.  The array is shared by every
  * call on one GPU: calls that overlap on one GPU race and must be
  * serialized by the caller.  Cores equal kc_decompose's. */

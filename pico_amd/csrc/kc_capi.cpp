@@ -683,8 +683,8 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         (s = ensure(g, g->kmax, 8, "kmax")))
         return s;
     uint32_t mode = method == KC_GPP ? 2 : method == KC_PEELONE ? 1 : 0;
-    // GPP: final cores; bucket peel: the live-vertex list of the high levels
-    if (mode != 1 && (s = ensure(g, g->out, (size_t)(g->V ? g->V : 1) * 4, "vertex list")))
+    // GPP: final cores (the cascading modes keep theirs in deg)
+    if (mode == 2 && (s = ensure(g, g->out, (size_t)(g->V ? g->V : 1) * 4, "vertex list")))
         return s;
     kc::PeelLaunch p{};
     p.rowptr = g->rowptr;
@@ -708,7 +708,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     // every level starts with a full scan)
     p.live_thresh = mode == 2 ? g->V / 2 : g->V / 4;
     // the dense core: pre-check + aggregation once at most V/agg_div remain
-    uint32_t agg_div = o.agg_div ? o.agg_div : 64;
+    uint32_t agg_div = o.agg_div ? o.agg_div : 128;
     p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : g->V / agg_div;
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;

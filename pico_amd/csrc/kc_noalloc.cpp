@@ -33,7 +33,7 @@ inline size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte layout of the work buffers in the array (offsets 256-aligned).
 struct Layout {
-    size_t gs, stats, done, live, list0, list1, total;
+    size_t gs, stats, done, list0, list1, total;
 };
 
 Layout layout_for(uint32_t V, uint32_t cap)
@@ -46,8 +46,6 @@ Layout layout_for(uint32_t V, uint32_t cap)
     o += up256(kc::KC_ST_COUNT * 8);
     l.done = o;
     o += up256((size_t)((V + 63ull) / 64) * 8);
-    l.live = o;
-    o += up256((size_t)(V ? V : 1) * 4);
     l.list0 = o;
     o += up256((size_t)cap * 8);
     l.list1 = o;
@@ -139,9 +137,8 @@ extern "C" kc_status kc_decompose_noalloc(kc_ctx *ctx, uint32_t V, uint64_t nnz,
     p.list1 = (uint64_t *)(s8 + l.list1);
     p.list_cap = (uint32_t)cap64;
     p.mode = 0;
-    p.out = (uint32_t *)(s8 + l.live);
-    p.live_thresh = V / 4;
-    const uint32_t agg_div = o.agg_div ? o.agg_div : 64;
+    p.out = nullptr; // the bucket peel keeps its cores in deg and no vertex list
+    const uint32_t agg_div = o.agg_div ? o.agg_div : 128;
     p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : V / agg_div;
     p.tail_live = 0; // one launch: the residual compaction needs allocations
     p.start_level = 0;
