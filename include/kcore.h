@@ -93,7 +93,10 @@ typedef struct kc_opts {
                              0xffffffff: never */
     uint32_t stage_grid; /* bucket peels and PeelOne: persistent workgroups for the compacted
                             stages (<= occupancy * CUs); 0: 3/8 of the largest resident grid */
-    uint32_t reserved[1];
+    uint32_t pull_div;   /* HistoCore: a round applies the hubs' updates by pulls (each hub
+                            scans its row against the round's changed vertices) when the
+                            round's walk exceeds 1/pull_div of the hubs' adjacency; 0: 3,
+                            0xffffffff: always */
 } kc_opts;
 
 typedef struct kc_stats {

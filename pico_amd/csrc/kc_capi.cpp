@@ -512,7 +512,8 @@ void kc_graph_free(kc_graph *g)
     }
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
-                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->hubs, &g->cs_row[0],
+                      &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->hubs, &g->hchg_bm,
+                      &g->hchg_rec, &g->cs_row[0],
                       &g->cs_row[1],
                       &g->cs_col[0], &g->cs_col[1], &g->cs_deg[0], &g->cs_deg[1], &g->cs_done[0],
                       &g->cs_done[1], &g->cs_gmap[0], &g->cs_gmap[1], &g->cs_loc, &g->cs_wpre,
@@ -983,6 +984,13 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
             return s;
         p.hubs = (uint32_t *)g->hubs.p;
         p.hub_cap = g->V;
+        size_t bmw = ((size_t)g->V + 63) / 64;
+        if ((s = ensure(g, g->hchg_bm, 2 * (bmw ? bmw : 1) * 8, "change bitmaps")) ||
+            (s = ensure(g, g->hchg_rec, (size_t)(g->V ? g->V : 1) * 8, "change records")))
+            return s;
+        p.chg_bm = (uint64_t *)g->hchg_bm.p;
+        p.chg_rec = (uint64_t *)g->hchg_rec.p;
+        p.pull_den = o.pull_div;
     }
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;

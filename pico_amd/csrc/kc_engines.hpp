@@ -38,6 +38,7 @@ enum {
 //  [10] WG 0 drain rounds                  [11] most edges walked by one WG
 //  peel only: [12] latest end of a WG's scan  [13] latest end of a WG's own
 //  cascade (before it helps with shared work)  [14] latest WG phase start
+//  HistoCore pull rounds: [12] latest end of a WG's hub pulls
 // Times are wall_clock64 ticks.
 enum { KC_TRACE_WORDS = 16 };
 
@@ -138,6 +139,9 @@ struct HindexLaunch {
     const uint32_t *hb_ds;  // histogram base of each col_ds neighbour, or null
     uint32_t *hubs;         // HistoCore hub lists (2 x hub_cap words), or null
     uint32_t hub_cap;
+    uint64_t *chg_bm;       // HistoCore pull rounds: 2 x ceil(V/64) words, or null
+    uint64_t *chg_rec;      // and V records (old << 32 | new)
+    uint32_t pull_den;      // pull when a round's walk > hub adjacency / pull_den (0: default)
     GridSync *gs;
     uint64_t *stats;
     uint32_t grid, block;

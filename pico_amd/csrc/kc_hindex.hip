@@ -61,6 +61,13 @@ template <typename R> struct HArgs {
     uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null;
                             // those of more than KC_HBIG also from hubs + hub_cap on
     uint32_t hub_cap;
+    // HistoCore pull rounds (with hubs): vertices changed in a count phase,
+    // one bit each, in two alternating bitmaps of bm_words words (round & 1),
+    // and each changed vertex's (old << 32 | new) value
+    uint64_t *chg_bm;
+    uint64_t *chg_rec;
+    uint32_t bm_words;
+    uint32_t pull_den; // pull when the round's walk exceeds hub adjacency / pull_den
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
@@ -73,13 +80,15 @@ constexpr uint32_t KC_HAGG_LOG2 = 11;
 constexpr uint32_t KC_HAGG = 1u << KC_HAGG_LOG2;
 constexpr uint32_t KC_HAGG_PROBES = 8;
 constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggregated
-// HistoCore rebuild rounds: in a round whose changed vertices reach more than
-// KC_PULL_NUM/KC_PULL_DEN as many adjacency entries as all hubs (vertices of
-// more than KC_PULL_T neighbours) hold, every hub's bins are recounted from its
-// neighbours' values (reads) instead of patched by two atomics per affected
-// edge; updates aimed at the other vertices are patched as usual.
+// HistoCore pull rounds: in a round whose changed vertices reach more than
+// 1/pull_den as many adjacency entries as all hubs (vertices of more than
+// KC_PULL_T neighbours) hold, every hub applies the updates of its changed
+// neighbours itself -- it scans its row against the round's bitmap of changed
+// vertices and sums the updates in LDS -- instead of receiving two atomics per
+// affected edge; updates aimed at the other vertices are patched as usual.
 constexpr uint32_t KC_PULL_T = 512;
-constexpr uint64_t KC_PULL_NUM = 1, KC_PULL_DEN = 3;
+constexpr uint32_t KC_PULL_DEN = 3; // default pull_den
+constexpr int KC_PULL_U = 2;        // row entries in flight per lane in a hub pull (more spill)
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
 template <int BLOCK> struct HLds {
@@ -94,7 +103,9 @@ template <int BLOCK> struct HLds {
     uint32_t red[BLOCK / KC_WAVE];
     uint32_t bk, bsum, bq; // block-level h-index result
     uint32_t stv[BLOCK / KC_WAVE][KC_HGRAB]; // per-wave staging of changed vertices
-    uint32_t stn[BLOCK / KC_WAVE][KC_HGRAB]; // and their virtual-node counts
+    uint32_t stn[BLOCK / KC_WAVE][KC_HGRAB]; // and their virtual-node counts,
+    uint32_t sto[BLOCK / KC_WAVE][KC_HGRAB]; // old values
+    uint32_t stk[BLOCK / KC_WAVE][KC_HGRAB]; // and new values
     uint32_t done;
 };
 
@@ -256,7 +267,10 @@ __device__ __forceinline__ uint64_t wave_count_gt(const uint32_t *nd, uint64_t n
 // HBM bins; the caller fences afterwards.  Values at or above cap (in the
 // dense core, most of a hub's neighbours) are counted in a register and
 // added to the top bin once, instead of one same-address atomic each.
-template <typename R, bool LDSB>
+// DIRECT: val runs parallel to the adjacency (val[i] is entry i's value, e.g.
+// the neighbour degrees beside the degree-ordered rows, whose row r holds the
+// same multiset as row r of col) instead of being indexed by neighbour id.
+template <typename R, bool LDSB, bool DIRECT = false>
 __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_t cap,
                                           const uint32_t *val, uint32_t *bins)
 {
@@ -271,7 +285,7 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint64_t ii = i + (uint64_t)k * KC_WAVE;
-            x[k] = ii < e ? val[a.col[ii]] : 0xffffffffu;
+            x[k] = ii < e ? (DIRECT ? val[ii] : val[a.col[ii]]) : 0xffffffffu;
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -293,9 +307,10 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
 // cap >= KC_LBINS, by one wave: values below KC_LBINS are counted in the
 // wave's LDS bins and stored once, only larger ones (few: neighbours of high
 // core) become HBM atomics.  bins[0] is not written.
+// direct: when not null, entry i's value is direct[i] (as wave_hist's DIRECT).
 template <typename R, int BLOCK>
 __device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
-                              uint32_t *bins, uint64_t &n_atom)
+                              uint32_t *bins, uint64_t &n_atom, const uint32_t *direct = nullptr)
 {
     uint32_t lane = lane_id();
     uint32_t *lb = L.bins[threadIdx.x / KC_WAVE];
@@ -312,7 +327,7 @@ __device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uin
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint64_t ii = i + (uint64_t)k * KC_WAVE;
-            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+            x[k] = ii < e ? (direct ? direct[ii] : a.core[a.col[ii]]) : 0xffffffffu;
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -346,7 +361,8 @@ __device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uin
 // one (NB words); values from NB up go to HBM atomics.  Returns bins[cap].
 template <typename R, int BLOCK>
 __device__ uint32_t block_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
-                                   uint32_t *bins, uint64_t &n_atom)
+                                   uint32_t *bins, uint64_t &n_atom,
+                                   const uint32_t *direct = nullptr)
 {
     constexpr uint32_t NB = (BLOCK / KC_WAVE) * KC_LBINS;
     uint32_t *fb = &L.bins[0][0];
@@ -365,7 +381,7 @@ __device__ uint32_t block_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint64_t ii = i + (uint64_t)k * BLOCK;
-            x[k] = ii < e ? a.core[a.col[ii]] : 0xffffffffu;
+            x[k] = ii < e ? (direct ? direct[ii] : a.core[a.col[ii]]) : 0xffffffffu;
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -510,33 +526,115 @@ __device__ uint32_t block_hsearch(HLds<BLOCK> &L, uint32_t *bins, uint32_t top,
     return 0;
 }
 
-// ---------------------------------------------- HistoCore hub rebuilds
-// Recount bins 1..cu of u's histogram (HBM words from rowptr[u]+u) as the
-// histogram of min(core(w), cu) over its neighbours w; bin 0 is kept.
-// Returns the top bin (neighbours at or above cu).  One wave.
-template <typename R, int BLOCK>
-__device__ __forceinline__ uint32_t wave_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u,
-                                                 uint32_t d, uint32_t cu, uint64_t &n_atom)
+// ---------------------------------------------- HistoCore hub pulls
+// A changed vertex's record for the pull rounds: its bit in the round's bitmap
+// and its (old << 32 | new) value.
+template <typename R>
+__device__ __forceinline__ void note_change(const HArgs<R> &a, uint32_t round, uint32_t v,
+                                            uint32_t old, uint32_t k)
 {
-    uint32_t lane = lane_id();
-    uint32_t w = threadIdx.x / KC_WAVE;
-    uint32_t *hb = a.histo + hbase(a.rowptr, u);
-    if (cu < KC_LBINS) {
-        wave_hist<R, true>(a, u, cu, a.core, L.bins[w]);
-        for (uint32_t j = 1 + lane; j <= cu; j += KC_WAVE)
-            hb[j] = L.bins[w][j];
-        return L.bins[w][cu];
-    }
-    wave_hist_hyb<R, BLOCK>(L, a, u, cu, hb, n_atom);
-    return ld_relaxed(&hb[cu]);
+    if (!a.chg_bm)
+        return;
+    a.chg_rec[v] = (uint64_t)old << 32 | k;
+    atomicOr((unsigned long long *)&a.chg_bm[(uint64_t)(round & 1) * a.bm_words + (v >> 6)],
+             1ull << (v & 63));
 }
 
-// The same for the whole workgroup (LDS bins of all its waves as one).
-template <typename R, int BLOCK>
-__device__ uint32_t block_rebuild(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t u, uint32_t cu,
-                                  uint64_t &n_atom)
+// One hub u (core cu) in a pull round.  Its neighbours whose bit is set in
+// cb changed in this round's count phase, from oc to nc (chg_rec); each one
+// with nc < cu moves one count of u's histogram from bin min(cu, oc) to bin
+// nc -- the update the push walk applies for that edge (the graph is
+// symmetric, so u's row lists exactly the changed vertices whose rows list
+// u).  The moves are summed in LDS (bins below NB; higher bins, which only
+// neighbours of value >= NB reach, take HBM atomics) and added to u's HBM bins
+// once.  No other group writes u's bins in this phase -- the walk skips hub
+// targets in a pull round -- so the additions are plain read-modify-writes.
+// Returns whether the top bin ended below cu: it was at least cu when the
+// phase began and only falls, so this is the flag the push walk would raise.
+// BLK: the whole workgroup (LDS bins of all its waves as one), else one wave.
+template <typename R, int BLOCK, bool BLK>
+__device__ bool hub_pull(HLds<BLOCK> &L, const HArgs<R> &a, const uint64_t *cb, uint32_t u,
+                         uint32_t cu, uint64_t &n_atom)
 {
-    return block_hist_hyb<R, BLOCK>(L, a, u, cu, a.histo + hbase(a.rowptr, u), n_atom);
+    constexpr uint32_t GS = BLK ? BLOCK : KC_WAVE;
+    constexpr uint32_t NB = BLK ? (BLOCK / KC_WAVE) * KC_LBINS : KC_LBINS;
+    const uint32_t t = BLK ? threadIdx.x : lane_id();
+    int32_t *lb = BLK ? (int32_t *)&L.bins[0][0] : (int32_t *)L.bins[threadIdx.x / KC_WAVE];
+    const uint32_t nl = cu < NB ? cu + 1 : NB; // LDS words in use
+    for (uint32_t j = t; j < nl; j += GS)
+        lb[j] = 0;
+    if (BLK)
+        __syncthreads();
+    else
+        __threadfence_block();
+    uint32_t *hb = a.histo + hbase(a.rowptr, u);
+    const uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+    uint32_t nhbm = 0;
+    for (uint64_t i = s + t; i < e; i += KC_PULL_U * (uint64_t)GS) {
+        uint32_t w[KC_PULL_U];
+        uint32_t bw[KC_PULL_U];
+        uint64_t rec[KC_PULL_U];
+        bool ch[KC_PULL_U];
+#pragma unroll
+        for (int k = 0; k < KC_PULL_U; k++) {
+            uint64_t ii = i + (uint64_t)k * GS;
+            w[k] = ii < e ? a.col[ii] : 0xffffffffu;
+        }
+        // the bitmap read as 32-bit words (bit v of word v >> 5)
+#pragma unroll
+        for (int k = 0; k < KC_PULL_U; k++)
+            bw[k] = w[k] != 0xffffffffu ? ((const uint32_t *)cb)[w[k] >> 5] : 0u;
+#pragma unroll
+        for (int k = 0; k < KC_PULL_U; k++) {
+            ch[k] = (bw[k] >> (w[k] & 31)) & 1u;
+            rec[k] = ch[k] ? a.chg_rec[w[k]] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < KC_PULL_U; k++) {
+            if (!ch[k])
+                continue;
+            const uint32_t oc = (uint32_t)(rec[k] >> 32), nc = (uint32_t)rec[k];
+            if (nc >= cu)
+                continue;
+            const uint32_t sb = oc < cu ? oc : cu;
+            if (sb < NB) {
+                atomicAdd(&lb[sb], -1);
+            } else {
+                atomicSub(&hb[sb], 1u);
+                nhbm++;
+            }
+            if (nc < NB) {
+                atomicAdd(&lb[nc], 1);
+            } else {
+                atomicAdd(&hb[nc], 1u);
+                nhbm++;
+            }
+        }
+    }
+    n_atom += nhbm;
+    vm_drain(); // this thread's HBM atomics are done
+    if (BLK)
+        __syncthreads();
+    else
+        __threadfence_block();
+    bool flag = false;
+    for (uint32_t j = t; j < nl; j += GS) {
+        const int32_t d = lb[j];
+        if (d) {
+            const uint32_t nv = hb[j] + (uint32_t)d;
+            hb[j] = nv;
+            if (j == cu)
+                flag = nv < cu;
+        }
+    }
+    if (cu >= NB) { // the top bin took HBM atomics from the whole group
+        if (BLK)
+            __syncthreads();
+        flag = t == 0 && ld_relaxed(&hb[cu]) < cu;
+    }
+    if (BLK)
+        return __syncthreads_or(flag) != 0;
+    return ballot(flag) != 0;
 }
 
 // One item index for the whole workgroup from counter ctr (n: item count;
@@ -737,16 +835,26 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 }
                 if (valid && !big) {
                     uint64_t s0 = a.rowptr[v], b = s0 + v;
+                    // the values are the neighbours' degrees: read them from
+                    // the degree-ordered rows when built (sequential), else
+                    // gather them through the adjacency
                     uint32_t x[KC_TINY];
+                    if (a.nd_ds) {
+#pragma unroll
+                        for (uint32_t k = 0; k < KC_TINY; k++)
+                            x[k] = k < d ? a.nd_ds[s0 + k] : 0u;
+                    } else {
+#pragma unroll
+                        for (uint32_t k = 0; k < KC_TINY; k++)
+                            x[k] = k < d ? a.col[s0 + k] : 0u;
+#pragma unroll
+                        for (uint32_t k = 0; k < KC_TINY; k++)
+                            if (k < d)
+                                x[k] = a.core[x[k]];
+                    }
 #pragma unroll
                     for (uint32_t k = 0; k < KC_TINY; k++)
-                        x[k] = k < d ? a.col[s0 + k] : 0u;
-#pragma unroll
-                    for (uint32_t k = 0; k < KC_TINY; k++)
-                        if (k < d) {
-                            uint32_t y = a.core[x[k]];
-                            x[k] = y < d ? y : d;
-                        }
+                        x[k] = x[k] < d ? x[k] : d;
                     a.histo[b] = d; // 0 for an isolated vertex, as the reference's zeroing
 #pragma unroll
                     for (uint32_t j = 1; j <= KC_TINY; j++) {
@@ -778,7 +886,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             uint32_t v = a.hubs[a.hub_cap + gi];
             uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
             uint64_t b = hbase(a.rowptr, v);
-            block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+            block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
             if (threadIdx.x == 0)
                 a.histo[b] = d;
             __syncthreads();
@@ -795,11 +903,14 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     return;
                 }
                 if (d < KC_LBINS) {
-                    wave_hist<R, true>(a, v, d, a.core, L.bins[wv]);
+                    if (a.nd_ds)
+                        wave_hist<R, true, true>(a, v, d, a.nd_ds, L.bins[wv]);
+                    else
+                        wave_hist<R, true>(a, v, d, a.core, L.bins[wv]);
                     for (uint32_t j = lane; j <= d; j += KC_WAVE)
                         a.histo[b + j] = j == 0 ? d : L.bins[wv][j];
                 } else {
-                    wave_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+                    wave_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
                     if (lane == 0)
                         a.histo[b] = d;
                 }
@@ -809,7 +920,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     return; // every item of more than KC_HBIG neighbours was built above
                 uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
                 uint64_t b = hbase(a.rowptr, v);
-                block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom);
+                block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
                 if (threadIdx.x == 0)
                     a.histo[b] = d;
                 __syncthreads();
@@ -844,7 +955,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             // nodes reserved with a single atomic per grab.
             uint32_t n_st = 0, st_nch = 0;
             auto one = [&](uint32_t v, uint32_t d) {
-                if (round == 1 && a.hubs && d > KC_HBIG)
+                if (a.hubs && d > KC_HBIG)
                     return; // counted first, below
                 uint32_t old = a.core[v];
                 if (round == 1 && old <= 1)
@@ -863,6 +974,8 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     a.core[v] = k;
                     L.stv[wv][n_st] = v;
                     L.stn[wv][n_st] = nch;
+                    L.sto[wv][n_st] = old;
+                    L.stk[wv][n_st] = k;
                     chg_deg += d;
                 }
                 n_st++;
@@ -873,6 +986,10 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     return;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                // the grab's change records, a lane each: one wait for all
+                // of them instead of one per changed vertex
+                if (lane < n_st)
+                    note_change(a, round, L.stv[wv][lane], L.sto[wv][lane], L.stk[wv][lane]);
                 uint32_t base = 0;
                 if (lane == 0) {
                     atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0, (unsigned long long)n_st);
@@ -917,6 +1034,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                         a.histo[b] = old;
                         a.histo[b + k] = sum;
                         a.core[v] = k;
+                        note_change(a, round, v, old, k);
                         chg_deg += d;
                         atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum0,
                                   1ull);
@@ -933,13 +1051,23 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 }
                 __syncthreads();
             };
-            if (round == 1) { // the largest rows first, a workgroup each
-                for (;;) {
-                    uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
-                    if (gi >= n_giants)
-                        break;
-                    big_count(a.hubs[a.hub_cap + gi]);
+            // The largest rows first, a workgroup each: in round 1 all of
+            // them, later those in the frontier -- whose top bin fell below
+            // their core (a flagged vertex's top bin ends below its core, and
+            // every other vertex's is at least its core).  Flags raised by
+            // the hub pulls come in bursts of such rows, which would pile up
+            // on the few workgroups whose grabs met them in the list.
+            for (;;) {
+                uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
+                if (gi >= n_giants)
+                    break;
+                uint32_t u = a.hubs[a.hub_cap + gi];
+                if (round > 1) {
+                    uint32_t cu = a.core[u];
+                    if (a.histo[hbase(a.rowptr, u) + cu] >= cu)
+                        continue;
                 }
+                big_count(u);
             }
             if (round == 1) {
                 // the first count of the vertices of at most KC_TINY
@@ -979,6 +1107,8 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                                 a.histo[b] = old;
                                 a.histo[b + k] = sum;
                                 a.core[v] = k;
+                                if (a.chg_rec)
+                                    a.chg_rec[v] = (uint64_t)old << 32 | k;
                                 chg = true;
                                 chg_deg += d;
                                 uint32_t r = d;
@@ -994,6 +1124,13 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     uint64_t cm = ballot(chg), qm = ballot(q);
                     uint32_t base = 0;
                     if (lane == 0) {
+                        // v0 is a multiple of 64: the wave's vertices are
+                        // one word of the change bitmap
+                        if (cm && a.chg_bm)
+                            atomicOr((unsigned long long *)&a.chg_bm[(uint64_t)(round & 1) *
+                                                                         a.bm_words +
+                                                                     (v0 >> 6)],
+                                     (unsigned long long)cm);
                         if (cm)
                             atomicAdd((unsigned long long *)&ps->sum0,
                                       (unsigned long long)__popcll(cm));
@@ -1012,7 +1149,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             }
             for_items_g<BLOCK, R>(L, a, phase, n_in, in_list, one,
                                 [&](uint32_t v) {
-                                    if (round == 1 && a.hubs)
+                                    if (a.hubs)
                                         return; // counted first, above
                                     big_count(v);
                                 },
@@ -1108,19 +1245,30 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             // nodes of the changed vertices: each wave takes KC_HGRAB nodes of
             // KC_HCH edges from the shared counter
             uint32_t *f_list = a.list[0];
-            const bool pull = n_hubs && a.nd_ds &&
-                              (uint64_t)n_chunks * KC_HCH * KC_PULL_DEN >
-                                  hub_edges * KC_PULL_NUM;
+            const bool pull = n_hubs && a.chg_bm &&
+                              (uint64_t)n_chunks * KC_HCH * a.pull_den > hub_edges;
             rebuilds += pull ? 1 : 0;
+            if (a.chg_bm) { // next round's bitmap: last read in the previous round
+                uint64_t *nb = a.chg_bm + (uint64_t)((round + 1) & 1) * a.bm_words;
+                for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < a.bm_words;
+                     i += gridDim.x * BLOCK)
+                    nb[i] = 0;
+            }
             if (pull) {
-                // Rebuild every hub's bins 1..core (bin 0 keeps the value
-                // the last collapse stored; bins above core are left as
-                // they are).  After a patch round bin j < core counts the
-                // neighbours of value j and bin core those at or above it,
-                // and the top bin was >= core when the phase began, so the
-                // recount is word for word the patched histogram and the
-                // hub is flagged exactly when its top bin ends below core.
-                // Hubs go first, so the edge walk below fills the tail.
+                // Every hub applies its changed neighbours' updates itself
+                // (hub_pull); the walk below leaves hub targets alone.  Rows
+                // of more than KC_HBIG neighbours first, a workgroup each, then
+                // the other hubs a wave each, so the edge walk fills the tail.
+                const uint64_t *cb = a.chg_bm + (uint64_t)(round & 1) * a.bm_words;
+                for (;;) {
+                    uint32_t gi = giant_grab(L, (uint32_t *)&slot_of(gs, phase)[0].sum1, n_giants);
+                    if (gi >= n_giants)
+                        break;
+                    uint32_t u = a.hubs[a.hub_cap + gi];
+                    bool f = hub_pull<R, BLOCK, true>(L, a, cb, u, a.core[u], n_atom);
+                    if (wv == 0)
+                        hpush(gs, phase, f_list, cap_list, lane == 0 && f, u);
+                }
                 if (threadIdx.x == 0)
                     L.nbig = 0;
                 __syncthreads();
@@ -1135,8 +1283,10 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     for (uint32_t it = base; it < end; it++) {
                         uint32_t u = a.hubs[it];
                         uint32_t d = (uint32_t)(a.rowptr[u + 1] - a.rowptr[u]);
+                        if (d > KC_HBIG)
+                            continue; // done above
                         uint32_t cu = a.core[u];
-                        if (d > KC_HBIG || cu >= KC_LBINS) { // to the workgroup
+                        if (cu >= KC_LBINS) { // most bins in HBM: to the workgroup
                             uint32_t slot = 0;
                             if (lane == 0)
                                 slot = atomicAdd(&L.nbig, 1u);
@@ -1147,20 +1297,23 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                                 continue;
                             }
                         }
-                        uint32_t top = wave_rebuild<R, BLOCK>(L, a, u, d, cu, n_atom);
-                        hpush(gs, phase, f_list, cap_list, lane == 0 && top < cu, u);
+                        bool f = hub_pull<R, BLOCK, false>(L, a, cb, u, cu, n_atom);
+                        hpush(gs, phase, f_list, cap_list, lane == 0 && f, u);
                     }
                 }
                 __syncthreads();
                 uint32_t nbig = L.nbig < KC_BIGQ ? L.nbig : KC_BIGQ;
                 for (uint32_t i = 0; i < nbig; i++) {
                     uint32_t u = L.bigq[i];
-                    uint32_t cu = a.core[u];
-                    uint32_t top = block_rebuild<R, BLOCK>(L, a, u, cu, n_atom);
+                    bool f = hub_pull<R, BLOCK, true>(L, a, cb, u, a.core[u], n_atom);
                     if (wv == 0)
-                        hpush(gs, phase, f_list, cap_list, lane == 0 && top < cu, u);
+                        hpush(gs, phase, f_list, cap_list, lane == 0 && f, u);
                     __syncthreads();
                 }
+                // trace: latest end of a workgroup's hub pulls
+                if (a.trace && phase < a.trace_cap && threadIdx.x == 0)
+                    atomicMax((unsigned long long *)&a.trace[(uint64_t)phase * KC_TRACE_WORDS + 12],
+                              (unsigned long long)now_ticks());
             }
             // One affected edge (v: oc -> nc, neighbour u of value cu > nc):
             // patch u's bins, summed per workgroup for hubs and applied when
@@ -1216,8 +1369,18 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     uint64_t rs = a.rowptr[v], re = a.rowptr[v + 1];
                     m_s = rs + (uint64_t)(uint32_t)(item >> 32) * KC_HCH;
                     m_e = m_s + KC_HCH < re ? m_s + KC_HCH : re;
-                    m_nc = a.core[v];
-                    m_oc = a.histo[rs + v];
+                    // a node of hub targets only (the row is ordered by
+                    // degree, largest first) has nothing left in a pull round
+                    if (pull && a.nd_ds[m_e - 1] > KC_PULL_T)
+                        m_e = m_s;
+                    if (a.chg_rec) {
+                        uint64_t r = a.chg_rec[v];
+                        m_nc = (uint32_t)r;
+                        m_oc = (uint32_t)(r >> 32);
+                    } else {
+                        m_nc = a.core[v];
+                        m_oc = a.histo[rs + v];
+                    }
                     upd_edges += m_e - m_s;
                 }
                 // KC_WALK_U virtual nodes (one wave-wide pass each) in flight
@@ -1246,7 +1409,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
 #pragma unroll
                     for (uint32_t j = 0; j < KC_WALK_U; j++) {
                         if (pull && nd[j] > KC_PULL_T)
-                            in[j] = false; // a rebuild round's hubs are done above
+                            in[j] = false; // a pull round's hubs pull themselves
                         cu[j] = in[j] ? a.core[u[j]] : 0u;
                     }
 #pragma unroll
@@ -1499,8 +1662,8 @@ out:
 
 template <typename R>
 __global__ void hindex_init_kernel(const R *rowptr, uint32_t V, uint32_t *core, uint32_t *cnt,
-                                   uint32_t *mark, GridSync *gs, uint32_t nblocks,
-                                   uint64_t timeout, uint64_t *stats)
+                                   uint32_t *mark, uint64_t *bm, uint32_t bm_len, GridSync *gs,
+                                   uint32_t nblocks, uint64_t timeout, uint64_t *stats)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
@@ -1511,6 +1674,8 @@ __global__ void hindex_init_kernel(const R *rowptr, uint32_t V, uint32_t *core, 
         if (mark)
             mark[v] = 0;
     }
+    for (uint32_t w = i; w < bm_len; w += stride)
+        bm[w] = 0;
     if (i == 0) {
         grid_sync_init(gs, nblocks, timeout);
         for (int k = 0; k < KC_ST_COUNT; k++)
@@ -1542,14 +1707,20 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     a.hb_ds = p.hb_ds;
     a.hubs = p.method == HM_HISTO ? p.hubs : nullptr;
     a.hub_cap = p.hub_cap;
+    // the pull rounds need the hub list and the degree-ordered rows
+    const bool pull_ok = a.hubs && p.nd_ds && p.chg_bm && p.chg_rec;
+    a.chg_bm = pull_ok ? p.chg_bm : nullptr;
+    a.chg_rec = pull_ok ? p.chg_rec : nullptr;
+    a.bm_words = (p.V + 63) / 64;
+    a.pull_den = p.pull_den ? p.pull_den : KC_PULL_DEN;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
     a.trace_cap = p.trace_cap;
     hipLaunchKernelGGL((hindex_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V,
                        p.core, p.method == HM_CNT ? p.cnt : nullptr,
-                       p.method != HM_HISTO ? p.mark : nullptr, p.gs, p.grid, p.timeout_ticks,
-                       p.stats);
+                       p.method != HM_HISTO ? p.mark : nullptr, a.chg_bm,
+                       a.chg_bm ? 2 * a.bm_words : 0u, p.gs, p.grid, p.timeout_ticks, p.stats);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

@@ -41,7 +41,7 @@ class Opts(ctypes.Structure):
                 ("gpp_compat", ctypes.c_uint32), ("part_size", ctypes.c_uint32),
                 ("local_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("agg_div", ctypes.c_uint32), ("compact_div", ctypes.c_uint32),
-                ("stage_grid", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
+                ("stage_grid", ctypes.c_uint32), ("pull_div", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):

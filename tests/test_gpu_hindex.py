@@ -181,3 +181,19 @@ def test_histocore_wave_rebuild_and_hub_aggregation(ctx, block):
     core, st = _check(ctx, rp, col, "histocore", hist=True, block=block)
     assert st["rebuild_rounds"] > 0, st
     assert int(core[:100].max()) == 99 and int(core[60100:60400].min()) == 299
+
+
+@pytest.mark.parametrize("pull_div", [1, 0, 12, 0xFFFFFFFF])
+@pytest.mark.parametrize("graph", ["rmat", "chunglu"])
+def test_histocore_pull_rounds_match_patches(ctx, pull_div, graph):
+    # hubs apply their changed neighbours' updates by a pull over their own
+    # row in some rounds (all of them with 0xffffffff, about none with 1); the
+    # cores, the rounds and every word of the final histogram are the same
+    # whichever rounds pull
+    if graph == "rmat":
+        hg = pa.HostGraph.rmat(15, 16, seed=11)
+    else:
+        hg = pa.HostGraph.chunglu(1 << 15, beta=2.1, mean_deg=20.0, cap=12000.0, seed=9)
+    _, st = _check(ctx, hg.rowptr, hg.col, "histocore", hist=True, pull_div=pull_div)
+    if pull_div == 0xFFFFFFFF:
+        assert st["rebuild_rounds"] > 0, st
