@@ -318,6 +318,10 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
         // undone).  In the dense core, read the degree first: dead targets
         // are skipped and decrements to vertices far above the level are
         // summed in LDS (flush_agg) instead of piling atomics on the hubs.
+        // Otherwise skip targets whose claim bit is set, read with a plain
+        // load (a stale word only lacks recent claims, so no live target is
+        // skipped): a decrement of a claimed vertex is always undone, and
+        // these are most of the edges once the first levels are over.
         const bool pc = L.precheck != 0;
         uint64_t pre[WALK_U];
 #pragma unroll
@@ -328,6 +332,10 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
                 // read a stale degree (stale values only ever lie above)
                 pre[k] = M == MODE_GPP ? ld_relaxed64(&a.done[u[k] >> 6])
                                             : (uint64_t)a.deg[u[k]];
+            else if (val[k]) // 0: claimed, never above the level
+                pre[k] = (((const uint32_t *)a.done)[u[k] >> 5] >> (u[k] & 31)) & 1u
+                             ? 0ull
+                             : 0xffffffffull;
         }
         uint32_t old[WALK_U];
         bool hit[WALK_U];
