@@ -558,15 +558,21 @@ __device__ void scan_slice(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
             uint32_t ik = i0 + NW * k;
             cc[k] = ik < n ? (use_list ? L.chunks[ik] : blockIdx.x + ik * nb) : KC_DEAD;
         }
+        // claim words and degrees in one round trip (a claimed vertex's
+        // degree is read and dropped); exact values: the cascades of
+        // earlier phases changed them by atomics
 #pragma unroll
-        for (uint32_t k = 0; k < U; k++)
+        for (uint32_t k = 0; k < U; k++) {
             dw[k] = cc[k] != KC_DEAD ? ld_relaxed64(&a.done[cc[k]]) : ~0ull;
+            uint32_t v = cc[k] * 64 + lane;
+            d[k] = cc[k] != KC_DEAD && v < a.V ? ld_relaxed(&a.deg[v]) : 0xffffffffu;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) {
             uint32_t v = cc[k] * 64 + lane;
             lv[k] = dw[k] != ~0ull && !((dw[k] >> lane) & 1ull) && v < a.V;
-            // exact value: the cascades of earlier phases changed it by atomics
-            d[k] = lv[k] ? ld_relaxed(&a.deg[v]) : 0xffffffffu;
+            if (!lv[k])
+                d[k] = 0xffffffffu;
         }
 #pragma unroll
         for (uint32_t k = 0; k < U; k++) {
