@@ -66,7 +66,7 @@ typedef enum kc_method {
 /* kc_opts.flags bits */
 #define KC_OPT_TRACE 1u       /* record a per-phase device trace (kc_graph_trace); the peel
                                  methods record it in the variant that also counts atomics */
-#define KC_OPT_NO_DEGSORT 2u  /* HistoCore: walk the adjacency in input order */
+#define KC_OPT_NO_DEGSORT 2u  /* h-index methods: walk the adjacency in input order */
 #define KC_OPT_COUNT 4u       /* peel methods: run the variant that counts its atomics
                                  (kc_stats.atomics); a little slower, so off by default */
 

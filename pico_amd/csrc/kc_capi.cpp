@@ -931,9 +931,10 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         chunk_cap = 0xffffffffull;
     if (method == KC_HISTO && (s = ensure(g, g->list1, (size_t)chunk_cap * 8, "virtual nodes")))
         return s;
-    // degree-ordered adjacency for HistoCore's update rounds, built once
-    bool use_ds = method == KC_HISTO && !(o.flags & KC_OPT_NO_DEGSORT) && g->nnz &&
-                  g->nnz <= 0x7fffffffull;
+    // degree-ordered adjacency, built once: it bounds HistoCore's update walks
+    // and CntCore's / NbrCore's marks and recounts, and gives every method its
+    // first round's values in row order
+    bool use_ds = !(o.flags & KC_OPT_NO_DEGSORT) && g->nnz && g->nnz <= 0x7fffffffull;
     if (use_ds && !g->degsorted) {
         size_t nb = (size_t)g->nnz * 4;
         if ((s = ensure(g, g->col_ds, nb, "ordered adjacency")) ||
@@ -979,7 +980,8 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
     p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
     p.hb_ds = use_ds && g->nnz + g->V < 0xffffffffull ? (const uint32_t *)g->hb_ds.p : nullptr;
-    if (use_ds) { // hub lists: all hubs, then those above 4,096 neighbours (V entries each)
+    if (use_ds && method == KC_HISTO) { // hub lists: all hubs, then those above 4,096
+                                        // neighbours (V entries each)
         if ((s = ensure(g, g->hubs, 2 * V4, "hub list")))
             return s;
         p.hubs = (uint32_t *)g->hubs.p;
@@ -991,6 +993,12 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
         p.chg_bm = (uint64_t *)g->hchg_bm.p;
         p.chg_rec = (uint64_t *)g->hchg_rec.p;
         p.pull_den = o.pull_div;
+    }
+    if (method == KC_NBR) { // NbrCore: its list of the largest rows
+        if ((s = ensure(g, g->hubs, V4, "giant list")))
+            return s;
+        p.giants = (uint32_t *)g->hubs.p;
+        p.giant_cap = g->V;
     }
     p.gs = (kc::GridSync *)g->gs.p;
     p.stats = (uint64_t *)g->stats.p;

@@ -68,6 +68,8 @@ template <typename R> struct HArgs {
     uint64_t *chg_rec;
     uint32_t bm_words;
     uint32_t pull_den; // pull when the round's walk exceeds hub adjacency / pull_den
+    uint32_t *giants;  // NbrCore: vertices of more than KC_HBIG neighbours, or null
+    uint32_t giant_cap;
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
@@ -928,6 +930,23 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
         if (!hbar(a, gs, gen, round, 0))
             goto out;
     }
+    // NbrCore: list the vertices of more than KC_HBIG neighbours once; every
+    // count phase runs those of them in its input first, a workgroup each,
+    // instead of leaving them to whichever workgroups' grabs met them in the
+    // list (which then finish the phase alone)
+    if (M == HM_NBR && a.giants) {
+        uint32_t phase = gen;
+        const uint32_t nthr = gridDim.x * BLOCK;
+        const uint32_t t0 = blockIdx.x * BLOCK + (threadIdx.x & ~(KC_WAVE - 1));
+        for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+            uint32_t v = v0 + lane;
+            bool big = v < V && a.rowptr[v + 1] - a.rowptr[v] > KC_HBIG;
+            hpush(gs, phase, a.giants, a.giant_cap, big, v);
+        }
+        if (!hbar(a, gs, gen, round, 0))
+            goto out;
+        n_giants = ld_relaxed(&slot_of(gs, phase)[0].qtail);
+    }
 
     for (;;) {
         round++;
@@ -1173,11 +1192,74 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 return true;
             };
             uint32_t n_st = 0;
+            // a vertex of more than KC_HBIG neighbours, by the workgroup
+            auto big_one = [&](uint32_t v) {
+                uint32_t old = a.core[v];
+                uint32_t sum;
+                uint64_t b = hbase(a.rowptr, v);
+                block_hist_hyb<R, BLOCK>(L, a, v, old, a.histo + b, n_atom);
+                uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
+                bool ch = k != old;
+                if (ch && threadIdx.x == 0)
+                    a.newc[v] = k;
+                if (!ch && M == HM_CNT && threadIdx.x == 0)
+                    a.cnt[v] = sum; // see cnt_refresh below
+                if (wv == 0)
+                    hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
+                __syncthreads();
+            };
+            const bool by_search = round == 1 && a.nd_ds;
+            if (M == HM_NBR && n_giants && !by_search) {
+                // the listed ones in this round's input first: the vertices
+                // marked in the last round (all of them in round 1)
+                for (;;) {
+                    uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
+                    if (gi >= n_giants)
+                        break;
+                    uint32_t v = a.giants[gi];
+                    if (a.mark[v] == round - 1)
+                        big_one(v);
+                }
+            }
+            if (by_search) {
+                // Round 1: every value is a degree, and a row of the
+                // degree-ordered adjacency holds its neighbours' degrees
+                // largest first, so the h-index of a vertex of degree d is
+                // the largest k <= d with nd[k-1] >= k: a binary search, a
+                // thread per vertex.  A vertex whose value holds (k = d, so
+                // all its d neighbours have degree >= d) has cnt = d.
+                const uint32_t nthr = gridDim.x * BLOCK;
+                const uint32_t t0 = blockIdx.x * BLOCK + (threadIdx.x & ~(KC_WAVE - 1));
+                for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+                    const uint32_t v = v0 + lane;
+                    bool ch = false;
+                    if (v < V) {
+                        const uint64_t s0 = a.rowptr[v];
+                        const uint32_t d = (uint32_t)(a.rowptr[v + 1] - s0);
+                        const uint32_t *nd = a.nd_ds + s0;
+                        uint32_t lo = 0, hi = d; // the answer lies in [lo, hi]
+                        while (lo < hi) {
+                            const uint32_t mid = hi - (hi - lo) / 2;
+                            if (nd[mid - 1] >= mid)
+                                lo = mid;
+                            else
+                                hi = mid - 1;
+                        }
+                        if (lo != d) {
+                            a.newc[v] = lo;
+                            ch = true;
+                        } else if (M == HM_CNT && d) {
+                            a.cnt[v] = d;
+                        }
+                    }
+                    hpush(gs, phase, c_list, cap_list, ch, v);
+                }
+            } else
             for_items_g<BLOCK, R>(
                 L, a, phase, n_in, in_list,
                 [&](uint32_t v, uint32_t d) {
-                    if (!want(v))
-                        return;
+                    if (!want(v) || (n_giants && d > KC_HBIG))
+                        return; // listed ones were counted first
                     uint32_t old = a.core[v], sum;
                     uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old, sum, n_atom);
                     if (k == old) {
@@ -1192,21 +1274,9 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     n_st++;
                 },
                 [&](uint32_t v) {
-                    if (!want(v))
-                        return;
-                    uint32_t old = a.core[v];
-                    uint32_t sum;
-                    uint64_t b = hbase(a.rowptr, v);
-                    block_hist_hyb<R, BLOCK>(L, a, v, old, a.histo + b, n_atom);
-                    uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
-                    bool ch = k != old;
-                    if (ch && threadIdx.x == 0)
-                        a.newc[v] = k;
-                    if (!ch && M == HM_CNT && threadIdx.x == 0)
-                        a.cnt[v] = sum; // see cnt_refresh below
-                    if (wv == 0)
-                        hpush(gs, phase, c_list, cap_list, ch && lane == 0, v);
-                    __syncthreads();
+                    if (n_giants || !want(v))
+                        return; // listed ones were counted first
+                    big_one(v);
                 },
                 [&]() { // one list reservation per grab for the staged vertices
                     if (n_st == 0)
@@ -1482,13 +1552,18 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     if (lane == 0)
                         a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    // core[u] <= deg(u): only the prefix of the degree-ordered
+                    // row with degree > nc can be affected
+                    if (a.nd_ds)
+                        e = s + wave_count_gt(a.nd_ds + s, d, nc);
+                    const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
                     for (uint64_t i0 = s; i0 < e; i0 += 4 * KC_WAVE) {
                         uint32_t u[4];
                         bool fresh[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             uint64_t i = i0 + (uint64_t)k * KC_WAVE + lane;
-                            u[k] = i < e ? a.col[i] : 0xffffffffu;
+                            u[k] = i < e ? adj[i] : 0xffffffffu;
                         }
                         // a mark read at this epoch is final (marks only
                         // grow): most neighbours of hubs skip the atomic
@@ -1512,13 +1587,16 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     if (threadIdx.x == 0)
                         a.core[v] = nc;
                     uint64_t s = a.rowptr[v], e = a.rowptr[v + 1];
+                    if (a.nd_ds) // the prefix of degree > nc, as above (each wave alike)
+                        e = s + wave_count_gt(a.nd_ds + s, (uint64_t)(e - s), nc);
+                    const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
                     for (uint64_t i0 = s; i0 < e; i0 += 4 * (uint64_t)BLOCK) {
                         uint32_t u[4];
                         bool fresh[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             uint64_t i = i0 + (uint64_t)k * BLOCK + threadIdx.x;
-                            u[k] = i < e ? a.col[i] : 0xffffffffu;
+                            u[k] = i < e ? adj[i] : 0xffffffffu;
                         }
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
@@ -1559,12 +1637,16 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     uint32_t cu = a.core[u];
                     uint32_t c = 0;
                     uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+                    // only neighbours of degree >= cu can hold a value >= cu
+                    if (a.nd_ds && cu)
+                        e = s + wave_count_gt(a.nd_ds + s, d, cu - 1);
+                    const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
                     for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
                         uint32_t w[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             uint64_t ii = i + (uint64_t)k * KC_WAVE;
-                            w[k] = ii < e ? a.col[ii] : 0xffffffffu;
+                            w[k] = ii < e ? adj[ii] : 0xffffffffu;
                         }
 #pragma unroll
                         for (int k = 0; k < 4; k++)
@@ -1579,12 +1661,15 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     uint32_t cu = a.core[u];
                     uint32_t c = 0;
                     uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
+                    if (a.nd_ds && cu) // the prefix of degree >= cu, as above (each wave alike)
+                        e = s + wave_count_gt(a.nd_ds + s, (uint64_t)(e - s), cu - 1);
+                    const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
                     for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
                         uint32_t w[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             uint64_t ii = i + (uint64_t)k * BLOCK;
-                            w[k] = ii < e ? a.col[ii] : 0xffffffffu;
+                            w[k] = ii < e ? adj[ii] : 0xffffffffu;
                         }
 #pragma unroll
                         for (int k = 0; k < 4; k++)
@@ -1713,6 +1798,9 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     a.chg_rec = pull_ok ? p.chg_rec : nullptr;
     a.bm_words = (p.V + 63) / 64;
     a.pull_den = p.pull_den ? p.pull_den : KC_PULL_DEN;
+    // (CntCore runs without: its register budget has no room for that path)
+    a.giants = p.method == HM_NBR ? p.giants : nullptr;
+    a.giant_cap = p.giant_cap;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
