@@ -16,10 +16,12 @@
 //    back or invalidate any L2, so a plain store would stay in its XCD's L2
 //    and a plain load could return a line cached before the barrier.  The
 //    read-only graph (rowptr, col) is the only data read with plain loads;
-//  * one planned exception: the peel's pre-check reads a degree with a plain
-//    load.  A stale value is never smaller than the true one (degrees only
-//    fall within a phase) and only decides whether to issue the atomic,
-//    whose returned value decides everything else;
+//  * two planned exceptions, both plain loads in the peel's edge walk that
+//    only decide whether to issue the atomic whose returned value decides
+//    everything else: the dense-core pre-check reads a degree (a stale value
+//    is never smaller than the true one: degrees only fall), and otherwise
+//    the walk reads the target's claim word (a stale word only lacks recent
+//    claims: claim bits are only ever set), so neither skips a live target;
 //  * grid_barrier<true> (agent-scope release/acquire fences, which write back
 //    and invalidate every XCD's L2) is for engines whose cross-workgroup data
 //    is written with plain stores.
