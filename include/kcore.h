@@ -122,8 +122,8 @@ typedef struct kc_stats {
                          arrays: degrees and claim bitmap (peel, with KC_OPT_COUNT;
                          0 without), histograms and marks (h-index, always); queue
                          counters are not included */
-    uint64_t rebuild_rounds; /* HistoCore: update rounds that recounted every hub's
-                                bins from its neighbours instead of patching them */
+    uint64_t rebuild_rounds; /* HistoCore: update rounds whose hubs pulled their changed
+                                neighbours' updates instead of receiving patches */
 } kc_stats;
 
 KC_API const char *kc_last_error(void);

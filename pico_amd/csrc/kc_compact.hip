@@ -177,6 +177,7 @@ __device__ void copy_long(uint32_t bid, uint32_t nblocks, const R *rowptr, const
     const uint32_t nlong = bad[1];
     constexpr uint32_t U = 16;
     constexpr uint32_t NW = CB / KC_WAVE;
+    static_assert(NW * U == KC_WAVE, "one count per lane of a wave");
     __shared__ uint32_t s_w[NW * U];
     __shared__ uint32_t s_pre[NW * U + 1];
     const uint32_t lane = lane_id(), w = threadIdx.x / KC_WAVE;
@@ -208,14 +209,14 @@ __device__ void copy_long(uint32_t bid, uint32_t nblocks, const R *rowptr, const
                     s_w[k * NW + w] = (uint32_t)__popcll(m);
             }
             __syncthreads();
-            // entries are ordered by (k, wave, lane): prefix over s_w in that order
-            if (threadIdx.x == 0) {
-                uint32_t run = 0;
-                for (uint32_t t = 0; t < NW * U; t++) {
-                    s_pre[t] = run;
-                    run += s_w[t];
-                }
-                s_pre[NW * U] = run;
+            // entries are ordered by (k, wave, lane): prefix over s_w in that
+            // order, its NW * U = 64 counts one per lane of wave 0
+            if (w == 0) {
+                const uint32_t x = s_w[lane];
+                const uint32_t incl = wave_incl_scan_u32(x);
+                s_pre[lane] = incl - x;
+                if (lane == KC_WAVE - 1)
+                    s_pre[NW * U] = incl;
             }
             __syncthreads();
 #pragma unroll

@@ -760,7 +760,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
     uint32_t gen = 0;
     uint32_t round = 0;
     uint64_t changed_total = 0;
-    uint64_t rebuilds = 0;  // HistoCore rounds that recounted the hubs
+    uint64_t rebuilds = 0;  // HistoCore rounds whose hubs pulled their updates
     uint64_t upd_edges = 0; // adjacency entries of changed vertices walked (HistoCore)
     uint64_t chg_deg = 0;   // degrees of changed vertices, summed (HistoCore)
     uint64_t n_atom = 0;    // atomics on graph-sized arrays (histograms, marks)
@@ -795,7 +795,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 bool valid = v < V;
                 uint32_t d = valid ? (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]) : 0u;
                 bool big = valid && d > KC_TINY;
-                // hubs, for the rebuild rounds: count in sum1, edges in sum2
+                // hubs, for the pull rounds: count in sum1, edges in sum2
                 bool hub = a.hubs && valid && d > KC_PULL_T;
                 uint64_t hm = ballot(hub);
                 if (hm) {
