@@ -824,7 +824,34 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint
 // and every reserved slot at or past the tail is dropped.  Ready items are always processed
 // before waiting on the rest of a reservation, so no workgroup sits on work
 // another one waits for.
-template <int BLOCK, int LQ, typename R, bool CNT, int M>
+// Level end without a grid barrier (bucket / PeelOne, fast instances): a
+// workgroup adds what it has gathered -- claims, the smallest degree left
+// above the level, the live flag -- to the phase's slot before each decrement
+// of the pending-work count (and waits for those atomics first), so once the
+// count reads 0 every contribution is in the slot.  The workgroup whose
+// decrement brings the count to 0 clears the slot of phase + 2 (read by every
+// workgroup at the end of phase - 1, before its own-work decrement of this
+// phase) before it does anything of phase + 1, whose end needs its own-work
+// decrement of that phase.  Thread 0.
+template <int BLOCK, int LQ>
+__device__ __forceinline__ void publish_phase(PeelLds<BLOCK, LQ> &L, PhaseSlot *s)
+{
+    if (L.claims) {
+        atomicAdd((unsigned long long *)&s->sum0, (unsigned long long)L.claims);
+        L.claims = 0;
+    }
+    if (L.minv != 0xffffffffu) {
+        atomicMin(&s->min0, L.minv);
+        L.minv = 0xffffffffu;
+    }
+    if (L.live) {
+        atomicOr(&s->flag, 1u);
+        L.live = 0;
+    }
+    vm_drain();
+}
+
+template <int BLOCK, int LQ, typename R, bool CNT, int M, bool FUSED>
 __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint32_t level,
                             uint32_t phase, uint64_t &edges)
 {
@@ -840,8 +867,14 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint
     if (threadIdx.x == 0) {
         // this WG's own work is over; if that was the last pending work, the
         // phase is over and nobody needs to look again
+        if (FUSED)
+            publish_phase(L, slot);
         unsigned long long old = atomicSub(qw, 1ull);
         L.hstate = (uint32_t)old == 1u ? 1u : 0u;
+        if (FUSED && (uint32_t)old == 1u) {
+            slot_clear(&a.gs->slot[(phase + 2) % 3], nb);
+            vm_drain();
+        }
     }
     __syncthreads();
     if (L.hstate == 1)
@@ -948,8 +981,15 @@ __device__ void help_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint
                 drain<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
                 vm_drain(); // the pushes these items caused come first
                 __syncthreads();
-                if (threadIdx.x == 0)
-                    atomicSub(qw, (unsigned long long)k);
+                if (threadIdx.x == 0) {
+                    if (FUSED)
+                        publish_phase(L, slot);
+                    unsigned long long old = atomicSub(qw, (unsigned long long)k);
+                    if (FUSED && (uint32_t)old == k) { // the phase's last work
+                        slot_clear(&a.gs->slot[(phase + 2) % 3], nb);
+                        vm_drain();
+                    }
+                }
             }
             // ---- anything left in the reservation?
             if (threadIdx.x == 0) {
@@ -1004,6 +1044,10 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R, M> a)
     __shared__ PeelLds<BLOCK, LQ> L;
     GridSync *gs = a.gs;
     const uint32_t nb = gridDim.x;
+    // bucket / PeelOne levels end on the shared queue's pending-work count
+    // (publish_phase); the diagnostic instance keeps the grid barrier, whose
+    // exit times its trace records
+    constexpr bool kFused = M != MODE_GPP && !CNT;
     {
         uint32_t nchunks = (a.V + 63) / 64;
         uint32_t cpw = blockIdx.x < nchunks ? (nchunks - blockIdx.x + nb - 1) / nb : 0;
@@ -1101,7 +1145,7 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R, M> a)
                 drain<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
                 if (tr && threadIdx.x == 0)
                     atomicMax((unsigned long long *)&tr[13], (unsigned long long)now_ticks());
-                help_shared<BLOCK, LQ, R, CNT, M>(L, a, level, phase, edges);
+                help_shared<BLOCK, LQ, R, CNT, M, kFused>(L, a, level, phase, edges);
             }
         } else if (kind == PH_SHARED) {
             take_shared<BLOCK, LQ, R, CNT, M>(L, a, level, phase, n_prev, TAKE_CHUNKS, edges);
@@ -1127,17 +1171,29 @@ __global__ __launch_bounds__(BLOCK) void peel_kernel(PeelArgs<R, M> a)
                 }
             }
         }
-        if (threadIdx.x == 0) {
-            PhaseSlot *s = &slot_of(gs, phase)[0];
-            if (L.claims)
-                atomicAdd((unsigned long long *)&s->sum0, (unsigned long long)L.claims);
-            if (L.minv != 0xffffffffu)
-                atomicMin(&s->min0, L.minv);
-            if (L.live)
-                atomicOr(&s->flag, 1u);
+        if constexpr (kFused) {
+            // help_shared returned with the phase over (every workgroup's
+            // contribution is in the slot) or with an error (hstate 2); one
+            // reading of the error word for the whole workgroup
+            if (threadIdx.x == 0 && ld_relaxed(&gs->err))
+                L.hstate = 2;
+            __syncthreads();
+            if (L.hstate == 2)
+                break;
+            gen++;
+        } else {
+            if (threadIdx.x == 0) {
+                PhaseSlot *s = &slot_of(gs, phase)[0];
+                if (L.claims)
+                    atomicAdd((unsigned long long *)&s->sum0, (unsigned long long)L.claims);
+                if (L.minv != 0xffffffffu)
+                    atomicMin(&s->min0, L.minv);
+                if (L.live)
+                    atomicOr(&s->flag, 1u);
+            }
+            if (!grid_barrier<false>(gs, gen))
+                break;
         }
-        if (!grid_barrier<false>(gs, gen))
-            break;
         PhaseSlot *s = &slot_of(gs, phase)[0];
         uint64_t claims = ld_relaxed64(&s->sum0);
         uint32_t n_next = M == MODE_GPP ? ld_relaxed(&s->qtail)
