@@ -825,8 +825,8 @@ __device__ void take_shared(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint
 // before waiting on the rest of a reservation, so no workgroup sits on work
 // another one waits for.
 // Level end without a grid barrier (bucket / PeelOne, fast instances): a
-// workgroup adds what it has gathered -- claims, the smallest degree left
-// above the level, the live flag -- to the phase's slot before each decrement
+// workgroup adds what it has gathered -- claims and the smallest degree left
+// above the level -- to the phase's slot before each decrement
 // of the pending-work count (and waits for those atomics first), so once the
 // count reads 0 every contribution is in the slot.  The workgroup whose
 // decrement brings the count to 0 clears the slot of phase + 2 (read by every
@@ -844,10 +844,7 @@ __device__ __forceinline__ void publish_phase(PeelLds<BLOCK, LQ> &L, PhaseSlot *
         atomicMin(&s->min0, L.minv);
         L.minv = 0xffffffffu;
     }
-    if (L.live) {
-        atomicOr(&s->flag, 1u);
-        L.live = 0;
-    }
+    // (the live flag is GPP's: its scans decide its last iteration)
     vm_drain();
 }
 
