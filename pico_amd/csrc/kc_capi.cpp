@@ -513,7 +513,7 @@ void kc_graph_free(kc_graph *g)
     for (DevBuf *b : {&g->deg, &g->out, &g->done, &g->list0, &g->list1, &g->gs, &g->stats,
                       &g->histo, &g->cnt, &g->flags, &g->hl0, &g->hl1, &g->hl2, &g->kmax,
                       &g->trace, &g->col_ds, &g->nd_ds, &g->hb_ds, &g->hubs, &g->hchg_bm,
-                      &g->hchg_rec, &g->cs_row[0],
+                      &g->hchg_rec, &g->hprep, &g->cs_row[0],
                       &g->cs_row[1],
                       &g->cs_col[0], &g->cs_col[1], &g->cs_deg[0], &g->cs_deg[1], &g->cs_done[0],
                       &g->cs_done[1], &g->cs_gmap[0], &g->cs_gmap[1], &g->cs_loc, &g->cs_wpre,
@@ -980,11 +980,15 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
     p.col_ds = use_ds ? (const uint32_t *)g->col_ds.p : nullptr;
     p.nd_ds = use_ds ? (const uint32_t *)g->nd_ds.p : nullptr;
     p.hb_ds = use_ds && g->nnz + g->V < 0xffffffffull ? (const uint32_t *)g->hb_ds.p : nullptr;
-    if (use_ds && method == KC_HISTO) { // hub lists: all hubs, then those above 4,096
-                                        // neighbours (V entries each)
-        if ((s = ensure(g, g->hubs, 2 * V4, "hub list")))
+    if (method == KC_HISTO) { // the rows above 4,096 neighbours (V entries), then the hubs
+        if ((s = ensure(g, g->hubs, 2 * V4, "hub list")) ||
+            (s = ensure(g, g->hprep, 64, "initial build")))
             return s;
-        p.hubs = (uint32_t *)g->hubs.p;
+        p.glist = (uint32_t *)g->hubs.p;
+        p.hprep = g->hprep.p;
+    }
+    if (use_ds && method == KC_HISTO) { // hubs, for the pull rounds
+        p.hubs = (uint32_t *)g->hubs.p + (g->V ? g->V : 1);
         p.hub_cap = g->V;
         size_t bmw = ((size_t)g->V + 63) / 64;
         if ((s = ensure(g, g->hchg_bm, 2 * (bmw ? bmw : 1) * 8, "change bitmaps")) ||

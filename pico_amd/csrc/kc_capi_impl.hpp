@@ -64,6 +64,7 @@ struct kc_graph {
     DevBuf col_ds, nd_ds, hb_ds; // degree-ordered adjacency (built on first HistoCore run)
     DevBuf hubs;                 // HistoCore's list of high-degree vertices
     DevBuf hchg_bm, hchg_rec;    // HistoCore pull rounds: change bitmaps and records
+    DevBuf hprep;                // HistoCore: counters of the initial build
     // bucket peel on compacted live subgraphs: two alternating stage sets
     DevBuf cs_row[2], cs_col[2], cs_deg[2], cs_done[2], cs_gmap[2], cs_loc, cs_wpre, cs_tmp, cs_bad;
     DevBuf aux; // small reductions (degree statistics)

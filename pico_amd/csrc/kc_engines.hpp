@@ -142,6 +142,9 @@ struct HindexLaunch {
     uint64_t *chg_bm;       // HistoCore pull rounds: 2 x ceil(V/64) words, or null
     uint64_t *chg_rec;      // and V records (old << 32 | new)
     uint32_t pull_den;      // pull when a round's walk > hub adjacency / pull_den (0: default)
+    uint32_t *glist;        // HistoCore: list of the vertices of more than 4,096
+                            // neighbours (V words)
+    void *hprep;            // HistoCore: the initial build's counters (64 bytes)
     uint32_t *giants;       // NbrCore: list of the vertices of more than 4,096
     uint32_t giant_cap;     // neighbours (giant_cap words), or null
     GridSync *gs;

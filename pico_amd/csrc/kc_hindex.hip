@@ -42,6 +42,20 @@ constexpr uint32_t KC_HCH = 64;     // edges per virtual node of a changed verte
 static_assert(KC_HCH == KC_WAVE, "a virtual node is one wave-wide pass");
 constexpr uint32_t KC_WALK_U = 2;   // virtual nodes in flight per wave in the update walk
 
+// Counters of HistoCore's initial build (histo_setup_kernel and the builders
+// after it), read by the persistent kernel when it starts.
+struct HistoPrep {
+    uint32_t n_big;    // vertices of more than KC_TINY neighbours, listed in list[3]
+    uint32_t n_hubs;   // hubs (more than KC_PULL_T neighbours), listed in hubs
+    uint32_t n_giants; // vertices of more than KC_HBIG neighbours, listed in glist
+    uint32_t err;      // a list overflowed
+    uint64_t hub_edges;
+    uint64_t atomics;  // HBM histogram atomics of the builders
+    uint32_t grab_giant; // work counter of the workgroup builder
+    uint32_t pad;
+};
+static_assert(sizeof(HistoPrep) <= 64, "the host allocates 64 bytes");
+
 template <typename R> struct HArgs {
     const R *rowptr;
     const uint32_t *col;
@@ -58,8 +72,7 @@ template <typename R> struct HArgs {
     const uint32_t *col_ds; // degree-ordered adjacency (kc_prep.hip) or null
     const uint32_t *nd_ds;  // neighbour degrees beside col_ds
     const uint32_t *hb_ds;  // neighbour histogram bases beside col_ds (nnz+V < 2^32), or null
-    uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null;
-                            // those of more than KC_HBIG also from hubs + hub_cap on
+    uint32_t *hubs;         // HistoCore: vertices of more than KC_PULL_T neighbours, or null
     uint32_t hub_cap;
     // HistoCore pull rounds (with hubs): vertices changed in a count phase,
     // one bit each, in two alternating bitmaps of bm_words words (round & 1),
@@ -70,6 +83,10 @@ template <typename R> struct HArgs {
     uint32_t pull_den; // pull when the round's walk exceeds hub adjacency / pull_den
     uint32_t *giants;  // NbrCore: vertices of more than KC_HBIG neighbours, or null
     uint32_t giant_cap;
+    // HistoCore: its list of the vertices of more than KC_HBIG neighbours
+    // (filled by histo_setup_kernel) and the counters of the initial build
+    uint32_t *glist;
+    HistoPrep *hp;
     GridSync *gs;
     uint64_t *stats;
     uint64_t *trace; // optional per-phase trace (KC_TRACE_WORDS per phase)
@@ -310,12 +327,12 @@ __device__ __forceinline__ void wave_hist(const HArgs<R> &a, uint32_t v, uint32_
 // wave's LDS bins and stored once, only larger ones (few: neighbours of high
 // core) become HBM atomics.  bins[0] is not written.
 // direct: when not null, entry i's value is direct[i] (as wave_hist's DIRECT).
-template <typename R, int BLOCK>
-__device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
+// lb: the wave's KC_LBINS words of LDS.
+template <typename R>
+__device__ void wave_hist_hyb(uint32_t *lb, const HArgs<R> &a, uint32_t v, uint32_t cap,
                               uint32_t *bins, uint64_t &n_atom, const uint32_t *direct = nullptr)
 {
     uint32_t lane = lane_id();
-    uint32_t *lb = L.bins[threadIdx.x / KC_WAVE];
     for (uint32_t j = lane; j < KC_LBINS; j += KC_WAVE)
         lb[j] = 0;
     for (uint32_t j = KC_LBINS + lane; j <= cap; j += KC_WAVE)
@@ -361,13 +378,13 @@ __device__ void wave_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uin
 
 // The same by the whole workgroup, its LDS bins being all waves' arrays as
 // one (NB words); values from NB up go to HBM atomics.  Returns bins[cap].
+// fb: the workgroup's (BLOCK / KC_WAVE) * KC_LBINS words of LDS.
 template <typename R, int BLOCK>
-__device__ uint32_t block_hist_hyb(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t cap,
+__device__ uint32_t block_hist_hyb(uint32_t *fb, const HArgs<R> &a, uint32_t v, uint32_t cap,
                                    uint32_t *bins, uint64_t &n_atom,
                                    const uint32_t *direct = nullptr)
 {
     constexpr uint32_t NB = (BLOCK / KC_WAVE) * KC_LBINS;
-    uint32_t *fb = &L.bins[0][0];
     const bool fits = cap < NB;
     const uint32_t lim = fits ? cap + 1 : NB; // LDS words in use
     for (uint32_t j = threadIdx.x; j < lim; j += BLOCK)
@@ -438,7 +455,7 @@ __device__ __forceinline__ uint32_t wave_hindex(HLds<BLOCK> &L, const HArgs<R> &
         return wave_hsearch(L.bins[w], cap, sum);
     }
     uint32_t *gb = a.histo + hbase(a.rowptr, v);
-    wave_hist_hyb<R, BLOCK>(L, a, v, cap, gb, n_atom);
+    wave_hist_hyb<R>(L.bins[w], a, v, cap, gb, n_atom);
     return wave_hsearch(gb, cap, sum);
 }
 
@@ -651,6 +668,92 @@ __device__ __forceinline__ uint32_t giant_grab(HLds<BLOCK> &L, uint32_t *ctr, ui
     return L.grab;
 }
 
+// HistoCore round 1 with the degree-ordered rows: every value is a degree
+// and a row of the degree-ordered adjacency holds its neighbours' degrees
+// largest first, so the search that would scan v's bins from its degree d
+// down is a binary search: k is the largest value <= d with nd[k-1] >= k, the
+// top bin after the collapse holds #{nd >= k} and the update walk covers
+// #{nd > k} entries.  A thread per vertex; the collapse, the change record
+// and the virtual nodes are those of the bin search.
+template <typename R>
+__device__ __forceinline__ void histo_round1_search(const HArgs<R> &a, uint32_t phase,
+                                                    uint64_t &chg_deg)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t V = a.V;
+    const uint32_t nthr = gridDim.x * blockDim.x;
+    const uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(KC_WAVE - 1));
+    PhaseSlot *ps = &slot_of(a.gs, phase)[0];
+    for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+        const uint32_t v = v0 + lane;
+        bool chg = false;
+        uint32_t nch = 0;
+        if (v < V) {
+            const uint64_t s0 = a.rowptr[v];
+            const uint32_t d = (uint32_t)(a.rowptr[v + 1] - s0);
+            const uint32_t *nd = a.nd_ds + s0;
+            uint32_t lo = 0, hi = d; // k lies in [lo, hi]
+            while (lo < hi) {
+                const uint32_t mid = hi - (hi - lo) / 2;
+                if (nd[mid - 1] >= mid)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            if (lo != d) {
+                const uint32_t k = lo;
+                uint32_t ge = k, gh = d; // entries >= k: the first k are
+                while (ge < gh) {
+                    const uint32_t mid = ge + (gh - ge) / 2;
+                    if (nd[mid] >= k)
+                        ge = mid + 1;
+                    else
+                        gh = mid;
+                }
+                uint32_t gt = 0, th = ge; // entries > k, among those >= k
+                while (gt < th) {
+                    const uint32_t mid = gt + (th - gt) / 2;
+                    if (nd[mid] > k)
+                        gt = mid + 1;
+                    else
+                        th = mid;
+                }
+                const uint64_t b = s0 + v;
+                a.histo[b] = d;
+                a.histo[b + k] = ge;
+                a.core[v] = k;
+                if (a.chg_rec)
+                    a.chg_rec[v] = (uint64_t)d << 32 | k;
+                chg = true;
+                chg_deg += d;
+                nch = (gt + KC_HCH - 1) / KC_HCH;
+            }
+        }
+        const uint64_t cm = ballot(chg);
+        const uint32_t incl = wave_incl_scan_u32(nch);
+        const uint32_t tot = __shfl(incl, KC_WAVE - 1, KC_WAVE);
+        uint32_t base = 0;
+        if (lane == 0) {
+            // v0 is a multiple of 64: one word of round 1's change bitmap
+            if (cm && a.chg_bm)
+                atomicOr((unsigned long long *)&a.chg_bm[(uint64_t)a.bm_words + (v0 >> 6)],
+                         (unsigned long long)cm);
+            if (cm)
+                atomicAdd((unsigned long long *)&ps->sum0, (unsigned long long)__popcll(cm));
+            if (tot)
+                base = atomicAdd(&ps->qtail, tot);
+        }
+        base = __shfl(base, 0, KC_WAVE) + incl - nch;
+        if (base + nch > a.chunk_cap) {
+            if (nch)
+                sync_fail(a.gs, 2);
+        } else {
+            for (uint32_t c = 0; c < nch; c++)
+                a.chunks[base + c] = (uint64_t)v | ((uint64_t)c << 32);
+        }
+    }
+}
+
 // ------------------------------------------------------------ grab loop
 // Runs f(item) for items [0, n) of a list (or the vertex range when lst is
 // null), one item per wave; items with more than KC_HBIG neighbours are
@@ -747,6 +850,164 @@ __device__ __forceinline__ bool hbar(const HArgs<R> &a, GridSync *gs, uint32_t &
     return ok;
 }
 
+// ------------------------------------------- HistoCore's initial build
+// histo_kernel (HistoCore.cu:15-75): bins of min(deg(v), deg(u)) over the
+// neighbours u.  Three ordinary launches before the persistent kernel, so that
+// none of their paths takes registers from it.
+//
+// A thread per vertex: list the hubs, the rows of more than KC_HBIG and those
+// of more than KC_TINY neighbours; build the histogram of a row of at most
+// KC_TINY neighbours in registers.
+template <typename R>
+__device__ __forceinline__ uint32_t prep_append(HistoPrep *hp, uint32_t *ctr, uint32_t *lst,
+                                                uint32_t cap, bool want, uint32_t v)
+{
+    uint64_t m = ballot(want);
+    if (!m)
+        return 0;
+    uint32_t lane = lane_id();
+    uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, KC_WAVE);
+    if (want) {
+        uint32_t p = base + lane_rank(m);
+        if (p < cap)
+            lst[p] = v;
+        else
+            atomicOr(&hp->err, 1u);
+    }
+    return (uint32_t)__popcll(m);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void histo_setup_kernel(HArgs<R> a)
+{
+    HistoPrep *hp = a.hp;
+    const uint32_t lane = lane_id();
+    const uint32_t V = a.V;
+    const uint32_t cap_list = V ? V : 1;
+    const uint32_t nthr = gridDim.x * blockDim.x;
+    const uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(KC_WAVE - 1));
+    for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
+        const uint32_t v = v0 + lane;
+        const bool valid = v < V;
+        const uint32_t d = valid ? (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]) : 0u;
+        const bool big = valid && d > KC_TINY;
+        const bool hub = a.hubs && valid && d > KC_PULL_T;
+        prep_append<R>(hp, &hp->n_big, a.list[3], cap_list, big, v);
+        prep_append<R>(hp, &hp->n_giants, a.glist, cap_list, valid && d > KC_HBIG, v);
+        if (prep_append<R>(hp, &hp->n_hubs, a.hubs, a.hub_cap, hub, v)) {
+            uint64_t hd = wave_sum_u64(hub ? (uint64_t)d : 0ull);
+            if (lane == 0)
+                atomicAdd((unsigned long long *)&hp->hub_edges, (unsigned long long)hd);
+        }
+        if (valid && !big) {
+            uint64_t s0 = a.rowptr[v], b = s0 + v;
+            // the values are the neighbours' degrees: read them from the
+            // degree-ordered rows when built (sequential), else gather them
+            // through the adjacency
+            uint32_t x[KC_TINY];
+            if (a.nd_ds) {
+#pragma unroll
+                for (uint32_t k = 0; k < KC_TINY; k++)
+                    x[k] = k < d ? a.nd_ds[s0 + k] : 0u;
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < KC_TINY; k++)
+                    x[k] = k < d ? a.col[s0 + k] : 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < KC_TINY; k++)
+                    if (k < d)
+                        x[k] = a.core[x[k]]; // the degrees: hindex_init_kernel's values
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < KC_TINY; k++)
+                x[k] = x[k] < d ? x[k] : d;
+            a.histo[b] = d; // 0 for an isolated vertex, as the reference's zeroing
+#pragma unroll
+            for (uint32_t j = 1; j <= KC_TINY; j++) {
+                if (j > d)
+                    break;
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < KC_TINY; k++)
+                    c += (k < d && x[k] == j) ? 1u : 0u;
+                a.histo[b + j] = c;
+            }
+        }
+    }
+}
+
+// A wave per listed row of more than KC_TINY and at most KC_HBIG neighbours
+// (wave w takes rows w, w + waves, ...: hundreds of thousands of rows taken
+// from one counter would queue on its address): LDS bins below KC_LBINS, HBM
+// atomics above.
+constexpr int KC_PREP_BLOCK = 256;
+template <typename R>
+__global__ __launch_bounds__(KC_PREP_BLOCK) void histo_rows_kernel(HArgs<R> a)
+{
+    __shared__ uint32_t bins[KC_PREP_BLOCK / KC_WAVE][KC_LBINS];
+    HistoPrep *hp = a.hp;
+    const uint32_t lane = lane_id(), wv = threadIdx.x / KC_WAVE;
+    const uint32_t n = hp->n_big < a.V ? hp->n_big : a.V;
+    uint64_t n_atom = 0;
+    const uint32_t nwv = gridDim.x * (KC_PREP_BLOCK / KC_WAVE);
+    for (uint32_t i = blockIdx.x * (KC_PREP_BLOCK / KC_WAVE) + wv; i < n; i += nwv) {
+        const uint32_t v = a.list[3][i];
+        const uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+        if (d > KC_HBIG)
+            continue; // a workgroup's, below
+        const uint64_t b = hbase(a.rowptr, v);
+        if (d < KC_LBINS) {
+            if (a.nd_ds)
+                wave_hist<R, true, true>(a, v, d, a.nd_ds, bins[wv]);
+            else
+                wave_hist<R, true>(a, v, d, a.core, bins[wv]);
+            for (uint32_t j = lane; j <= d; j += KC_WAVE)
+                a.histo[b + j] = j == 0 ? d : bins[wv][j];
+        } else {
+            wave_hist_hyb<R>(bins[wv], a, v, d, a.histo + b, n_atom, a.nd_ds);
+            if (lane == 0)
+                a.histo[b] = d;
+        }
+    }
+    n_atom = wave_sum_u64(n_atom);
+    if (lane == 0 && n_atom)
+        atomicAdd((unsigned long long *)&hp->atomics, (unsigned long long)n_atom);
+}
+
+// A workgroup per row of more than KC_HBIG neighbours, its LDS bins as one.
+constexpr int KC_GIANT_BLOCK = 512;
+template <typename R>
+__global__ __launch_bounds__(KC_GIANT_BLOCK) void histo_giants_kernel(HArgs<R> a)
+{
+    __shared__ uint32_t fb[(KC_GIANT_BLOCK / KC_WAVE) * KC_LBINS];
+    __shared__ uint32_t s_i;
+    HistoPrep *hp = a.hp;
+    const uint32_t n = hp->n_giants < a.V ? hp->n_giants : a.V;
+    uint64_t n_atom = 0;
+    for (;;) {
+        __syncthreads(); // the previous index has been read by every thread
+        if (threadIdx.x == 0)
+            s_i = atomicAdd(&hp->grab_giant, 1u);
+        __syncthreads();
+        const uint32_t i = s_i;
+        if (i >= n)
+            break;
+        const uint32_t v = a.glist[i];
+        const uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+        const uint64_t b = hbase(a.rowptr, v);
+        block_hist_hyb<R, KC_GIANT_BLOCK>(fb, a, v, d, a.histo + b, n_atom, a.nd_ds);
+        if (threadIdx.x == 0)
+            a.histo[b] = d;
+    }
+    n_atom = wave_sum_u64(n_atom);
+    if (lane_id() == 0 && n_atom)
+        atomicAdd((unsigned long long *)&hp->atomics, (unsigned long long)n_atom);
+}
+
 // ---------------------------------------------------------------- kernel
 // One instance per method (M), so each carries only its own paths' registers.
 template <int BLOCK, typename R, uint32_t M>
@@ -776,159 +1037,24 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
     }
     __syncthreads();
 
-    // ---- HistoCore: initial histograms, bins min(deg(v), deg(u))
-    //      (histo_kernel, HistoCore.cu:15-75)
+    // HistoCore: the initial histograms were built by histo_setup_kernel and
+    // the builders after it, which also listed the hubs and the largest rows
     uint32_t n_big = V; // HistoCore: vertices of more than KC_TINY neighbours
     uint32_t n_hubs = 0;    // HistoCore: listed hubs and their adjacency entries
     uint64_t hub_edges = 0;
-    uint32_t n_giants = 0;  // HistoCore: listed vertices of more than KC_HBIG neighbours
+    uint32_t n_giants = 0;  // listed vertices of more than KC_HBIG neighbours
     if (M == HM_HISTO) {
-        // Most vertices of a power-law graph have a handful of neighbours: a
-        // thread builds such a histogram in registers and stores its d+1
-        // words; the others are listed for the wave / workgroup builders.
-        uint32_t phase = gen;
-        {
-            const uint32_t nthr = gridDim.x * BLOCK;
-            const uint32_t t0 = blockIdx.x * BLOCK + (threadIdx.x & ~(KC_WAVE - 1));
-            for (uint32_t v0 = t0; v0 < V; v0 += nthr) { // wave-uniform trip count
-                uint32_t v = v0 + lane;
-                bool valid = v < V;
-                uint32_t d = valid ? (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]) : 0u;
-                bool big = valid && d > KC_TINY;
-                // hubs, for the pull rounds: count in sum1, edges in sum2
-                bool hub = a.hubs && valid && d > KC_PULL_T;
-                uint64_t hm = ballot(hub);
-                if (hm) {
-                    uint64_t hd = wave_sum_u64(hub ? (uint64_t)d : 0ull);
-                    uint32_t leader = __ffsll((unsigned long long)hm) - 1;
-                    uint32_t hb = 0;
-                    if (lane == leader) {
-                        hb = (uint32_t)atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum1,
-                                                 (unsigned long long)__popcll(hm));
-                        atomicAdd((unsigned long long *)&slot_of(gs, phase)[0].sum2,
-                                  (unsigned long long)hd);
-                    }
-                    hb = __shfl(hb, leader, KC_WAVE);
-                    if (hub) {
-                        uint32_t p = hb + lane_rank(hm);
-                        if (p < a.hub_cap)
-                            a.hubs[p] = v;
-                        else
-                            sync_fail(gs, 2);
-                    }
-                    // the workgroup-sized ones again, counted in flag: their
-                    // first histogram and first count start each phase
-                    bool giant = hub && d > KC_HBIG;
-                    uint64_t gm = ballot(giant);
-                    if (gm) {
-                        uint32_t gl = __ffsll((unsigned long long)gm) - 1;
-                        uint32_t gb = 0;
-                        if (lane == gl)
-                            gb = atomicAdd(&slot_of(gs, phase)[0].flag, (uint32_t)__popcll(gm));
-                        gb = __shfl(gb, gl, KC_WAVE);
-                        if (giant) {
-                            uint32_t p = gb + lane_rank(gm);
-                            if (p < a.hub_cap)
-                                a.hubs[a.hub_cap + p] = v;
-                            else
-                                sync_fail(gs, 2);
-                        }
-                    }
-                }
-                if (valid && !big) {
-                    uint64_t s0 = a.rowptr[v], b = s0 + v;
-                    // the values are the neighbours' degrees: read them from
-                    // the degree-ordered rows when built (sequential), else
-                    // gather them through the adjacency
-                    uint32_t x[KC_TINY];
-                    if (a.nd_ds) {
-#pragma unroll
-                        for (uint32_t k = 0; k < KC_TINY; k++)
-                            x[k] = k < d ? a.nd_ds[s0 + k] : 0u;
-                    } else {
-#pragma unroll
-                        for (uint32_t k = 0; k < KC_TINY; k++)
-                            x[k] = k < d ? a.col[s0 + k] : 0u;
-#pragma unroll
-                        for (uint32_t k = 0; k < KC_TINY; k++)
-                            if (k < d)
-                                x[k] = a.core[x[k]];
-                    }
-#pragma unroll
-                    for (uint32_t k = 0; k < KC_TINY; k++)
-                        x[k] = x[k] < d ? x[k] : d;
-                    a.histo[b] = d; // 0 for an isolated vertex, as the reference's zeroing
-#pragma unroll
-                    for (uint32_t j = 1; j <= KC_TINY; j++) {
-                        if (j > d)
-                            break;
-                        uint32_t c = 0;
-#pragma unroll
-                        for (uint32_t k = 0; k < KC_TINY; k++)
-                            c += (k < d && x[k] == j) ? 1u : 0u;
-                        a.histo[b + j] = c;
-                    }
-                }
-                hpush(gs, phase, a.list[3], cap_list, big, v);
-            }
-        }
-        if (!hbar(a, gs, gen, round, 0))
+        if (a.hp->err) {
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                sync_fail(gs, 2);
             goto out;
-        n_big = ld_relaxed(&slot_of(gs, phase)[0].qtail);
-        n_hubs = (uint32_t)ld_relaxed64(&slot_of(gs, phase)[0].sum1);
-        hub_edges = ld_relaxed64(&slot_of(gs, phase)[0].sum2);
-        n_giants = ld_relaxed(&slot_of(gs, phase)[0].flag);
-        phase = gen;
-        // the largest rows first, a workgroup each, so that none of them is
-        // left for the end of the phase
-        for (;;) {
-            uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
-            if (gi >= n_giants)
-                break;
-            uint32_t v = a.hubs[a.hub_cap + gi];
-            uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
-            uint64_t b = hbase(a.rowptr, v);
-            block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
-            if (threadIdx.x == 0)
-                a.histo[b] = d;
-            __syncthreads();
         }
-        for_items<BLOCK, R>(
-            L, a, phase, n_big, a.list[3],
-            [&](uint32_t v, uint32_t d) {
-                if (a.hubs && d > KC_HBIG)
-                    return; // built above
-                uint64_t b = hbase(a.rowptr, v);
-                if (d == 0) { // the reference zeroes the whole array first (:286)
-                    if (lane == 0)
-                        a.histo[b] = 0;
-                    return;
-                }
-                if (d < KC_LBINS) {
-                    if (a.nd_ds)
-                        wave_hist<R, true, true>(a, v, d, a.nd_ds, L.bins[wv]);
-                    else
-                        wave_hist<R, true>(a, v, d, a.core, L.bins[wv]);
-                    for (uint32_t j = lane; j <= d; j += KC_WAVE)
-                        a.histo[b + j] = j == 0 ? d : L.bins[wv][j];
-                } else {
-                    wave_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
-                    if (lane == 0)
-                        a.histo[b] = d;
-                }
-            },
-            [&](uint32_t v) {
-                if (a.hubs)
-                    return; // every item of more than KC_HBIG neighbours was built above
-                uint32_t d = (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
-                uint64_t b = hbase(a.rowptr, v);
-                block_hist_hyb<R, BLOCK>(L, a, v, d, a.histo + b, n_atom, a.nd_ds);
-                if (threadIdx.x == 0)
-                    a.histo[b] = d;
-                __syncthreads();
-            });
-        if (!hbar(a, gs, gen, round, 0))
-            goto out;
+        n_big = a.hp->n_big;
+        n_giants = a.hp->n_giants;
+        if (a.hubs) {
+            n_hubs = a.hp->n_hubs;
+            hub_edges = a.hp->hub_edges;
+        }
     }
     // NbrCore: list the vertices of more than KC_HBIG neighbours once; every
     // count phase runs those of them in its input first, a workgroup each,
@@ -974,7 +1100,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             // nodes reserved with a single atomic per grab.
             uint32_t n_st = 0, st_nch = 0;
             auto one = [&](uint32_t v, uint32_t d) {
-                if (a.hubs && d > KC_HBIG)
+                if (d > KC_HBIG)
                     return; // counted first, below
                 uint32_t old = a.core[v];
                 if (round == 1 && old <= 1)
@@ -1070,6 +1196,9 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 }
                 __syncthreads();
             };
+            if (round == 1 && a.nd_ds) {
+                histo_round1_search<R>(a, phase, chg_deg);
+            } else {
             // The largest rows first, a workgroup each: in round 1 all of
             // them, later those in the frontier -- whose top bin fell below
             // their core (a flagged vertex's top bin ends below its core, and
@@ -1080,7 +1209,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 uint32_t gi = giant_grab(L, &slot_of(gs, phase)[0].flag, n_giants);
                 if (gi >= n_giants)
                     break;
-                uint32_t u = a.hubs[a.hub_cap + gi];
+                uint32_t u = a.glist[gi];
                 if (round > 1) {
                     uint32_t cu = a.core[u];
                     if (a.histo[hbase(a.rowptr, u) + cu] >= cu)
@@ -1168,11 +1297,10 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             }
             for_items_g<BLOCK, R>(L, a, phase, n_in, in_list, one,
                                 [&](uint32_t v) {
-                                    if (a.hubs)
-                                        return; // counted first, above
-                                    big_count(v);
+                                    (void)v; // counted first, above
                                 },
                                 flush);
+            }
         } else {
             // NbrCore count_kernel (NbrCore.cu:78-113) / CntCore count_kernel
             // (CntCore.cu:82-130): recompute from scratch; the new value is
@@ -1197,7 +1325,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                 uint32_t old = a.core[v];
                 uint32_t sum;
                 uint64_t b = hbase(a.rowptr, v);
-                block_hist_hyb<R, BLOCK>(L, a, v, old, a.histo + b, n_atom);
+                block_hist_hyb<R, BLOCK>(&L.bins[0][0], a, v, old, a.histo + b, n_atom);
                 uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
                 bool ch = k != old;
                 if (ch && threadIdx.x == 0)
@@ -1334,7 +1462,7 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     uint32_t gi = giant_grab(L, (uint32_t *)&slot_of(gs, phase)[0].sum1, n_giants);
                     if (gi >= n_giants)
                         break;
-                    uint32_t u = a.hubs[a.hub_cap + gi];
+                    uint32_t u = a.glist[gi];
                     bool f = hub_pull<R, BLOCK, true>(L, a, cb, u, a.core[u], n_atom);
                     if (wv == 0)
                         hpush(gs, phase, f_list, cap_list, lane == 0 && f, u);
@@ -1718,6 +1846,9 @@ out:
         a.stats[KC_ST_CHANGED] = changed_total;
         a.stats[KC_ST_REBUILDS] = rebuilds;
         a.stats[KC_ST_ERR] = ld_relaxed(&gs->err);
+        if (M == HM_HISTO)
+            atomicAdd((unsigned long long *)&a.stats[KC_ST_ATOMICS],
+                      (unsigned long long)a.hp->atomics);
     }
     // per-workgroup sums first: one global atomic per counter and workgroup
     // (every path to here is taken by the whole workgroup)
@@ -1748,8 +1879,11 @@ out:
 template <typename R>
 __global__ void hindex_init_kernel(const R *rowptr, uint32_t V, uint32_t *core, uint32_t *cnt,
                                    uint32_t *mark, uint64_t *bm, uint32_t bm_len, GridSync *gs,
-                                   uint32_t nblocks, uint64_t timeout, uint64_t *stats)
+                                   uint32_t nblocks, uint64_t timeout, uint64_t *stats,
+                                   HistoPrep *hp)
 {
+    if (hp && blockIdx.x == 0 && threadIdx.x == 0)
+        *hp = HistoPrep{};
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = i; v < V; v += stride) {
@@ -1801,6 +1935,10 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     // (CntCore runs without: its register budget has no room for that path)
     a.giants = p.method == HM_NBR ? p.giants : nullptr;
     a.giant_cap = p.giant_cap;
+    a.glist = p.method == HM_HISTO ? p.glist : nullptr;
+    a.hp = p.method == HM_HISTO ? (HistoPrep *)p.hprep : nullptr;
+    if (p.method == HM_HISTO && (!a.glist || !a.hp))
+        return hipErrorInvalidValue;
     a.gs = p.gs;
     a.stats = p.stats;
     a.trace = p.trace;
@@ -1808,10 +1946,20 @@ static hipError_t launch_hindex_t(const HindexLaunch &p, hipStream_t st)
     hipLaunchKernelGGL((hindex_init_kernel<R>), dim3(256), dim3(256), 0, st, a.rowptr, p.V,
                        p.core, p.method == HM_CNT ? p.cnt : nullptr,
                        p.method != HM_HISTO ? p.mark : nullptr, a.chg_bm,
-                       a.chg_bm ? 2 * a.bm_words : 0u, p.gs, p.grid, p.timeout_ticks, p.stats);
+                       a.chg_bm ? 2 * a.bm_words : 0u, p.gs, p.grid, p.timeout_ticks, p.stats,
+                       a.hp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
+    if (p.method == HM_HISTO) { // the initial histograms, before the rounds
+        uint32_t gb = (p.V + 255) / 256;
+        hipLaunchKernelGGL(histo_setup_kernel<R>, dim3(gb < 16384 ? (gb ? gb : 1) : 16384),
+                           dim3(256), 0, st, a);
+        hipLaunchKernelGGL(histo_rows_kernel<R>, dim3(2048), dim3(KC_PREP_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(histo_giants_kernel<R>, dim3(256), dim3(KC_GIANT_BLOCK), 0, st, a);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
     void *args[] = {&a};
     const void *fn = p.method == HM_HISTO ? (const void *)hindex_kernel<BLOCK, R, HM_HISTO>
                      : p.method == HM_CNT ? (const void *)hindex_kernel<BLOCK, R, HM_CNT>
