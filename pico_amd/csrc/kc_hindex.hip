@@ -107,7 +107,7 @@ constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggrega
 // affected edge; updates aimed at the other vertices are patched as usual.
 constexpr uint32_t KC_PULL_T = 1024; // R-MAT s22: 512 29.8 ms, 1,024 28.2, 1,536 29.1
 constexpr uint32_t KC_PULL_DEN = 3; // default pull_den
-constexpr int KC_PULL_U = 2;        // row entries in flight per lane in a hub pull (more spill)
+constexpr int KC_PULL_U = 6;        // row entries in flight per lane in a hub pull (R-MAT s22: 2 27.2 ms, 4 26.6, 6 26.5, 8 26.7)
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
 template <int BLOCK> struct HLds {
@@ -577,6 +577,7 @@ __device__ bool hub_pull(HLds<BLOCK> &L, const HArgs<R> &a, const uint64_t *cb, 
 {
     constexpr uint32_t GS = BLK ? BLOCK : KC_WAVE;
     constexpr uint32_t NB = BLK ? (BLOCK / KC_WAVE) * KC_LBINS : KC_LBINS;
+    constexpr int U = BLOCK >= 1024 ? 2 : KC_PULL_U; // 1,024 threads: 128 VGPRs, no room
     const uint32_t t = BLK ? threadIdx.x : lane_id();
     int32_t *lb = BLK ? (int32_t *)&L.bins[0][0] : (int32_t *)L.bins[threadIdx.x / KC_WAVE];
     const uint32_t nl = cu < NB ? cu + 1 : NB; // LDS words in use
@@ -589,27 +590,27 @@ __device__ bool hub_pull(HLds<BLOCK> &L, const HArgs<R> &a, const uint64_t *cb, 
     uint32_t *hb = a.histo + hbase(a.rowptr, u);
     const uint64_t s = a.rowptr[u], e = a.rowptr[u + 1];
     uint32_t nhbm = 0;
-    for (uint64_t i = s + t; i < e; i += KC_PULL_U * (uint64_t)GS) {
-        uint32_t w[KC_PULL_U];
-        uint32_t bw[KC_PULL_U];
-        uint64_t rec[KC_PULL_U];
-        bool ch[KC_PULL_U];
+    for (uint64_t i = s + t; i < e; i += U * (uint64_t)GS) {
+        uint32_t w[U];
+        uint32_t bw[U];
+        uint64_t rec[U];
+        bool ch[U];
 #pragma unroll
-        for (int k = 0; k < KC_PULL_U; k++) {
+        for (int k = 0; k < U; k++) {
             uint64_t ii = i + (uint64_t)k * GS;
             w[k] = ii < e ? a.col[ii] : 0xffffffffu;
         }
         // the bitmap read as 32-bit words (bit v of word v >> 5)
 #pragma unroll
-        for (int k = 0; k < KC_PULL_U; k++)
+        for (int k = 0; k < U; k++)
             bw[k] = w[k] != 0xffffffffu ? ((const uint32_t *)cb)[w[k] >> 5] : 0u;
 #pragma unroll
-        for (int k = 0; k < KC_PULL_U; k++) {
+        for (int k = 0; k < U; k++) {
             ch[k] = (bw[k] >> (w[k] & 31)) & 1u;
             rec[k] = ch[k] ? a.chg_rec[w[k]] : 0ull;
         }
 #pragma unroll
-        for (int k = 0; k < KC_PULL_U; k++) {
+        for (int k = 0; k < U; k++) {
             if (!ch[k])
                 continue;
             const uint32_t oc = (uint32_t)(rec[k] >> 32), nc = (uint32_t)rec[k];
