@@ -92,7 +92,7 @@ typedef struct kc_opts {
                              subgraph, again at each further 1/compact_div; 0: default (8),
                              0xffffffff: never */
     uint32_t stage_grid; /* bucket peels and PeelOne: persistent workgroups for the compacted
-                            stages (<= occupancy * CUs); 0: 3/8 of the largest resident grid */
+                            stages (<= occupancy * CUs); 0: half of the largest resident grid */
     uint32_t pull_div;   /* HistoCore: a round applies the hubs' updates by pulls (each hub
                             scans its row against the round's changed vertices) when the
                             round's walk exceeds 1/pull_div of the hubs' adjacency; 0: 3,

@@ -738,9 +738,12 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     };
     p.tail_live = tail_of(g->V);
     p.start_level = 0;
+    // the whole graph's walks sum their decrements in LDS when compacted
+    // stages take over the dense levels (kc_peel.hip, walk_items)
+    p.agg_walks = mode == 0 && p.tail_live ? 1u : 0u;
     // workgroups of the compacted stages' launches (see below)
     const uint32_t stage_g =
-        std::max(1u, std::min(o.stage_grid ? o.stage_grid : maxgrid * 3 / 8, maxgrid));
+        std::max(1u, std::min(o.stage_grid ? o.stage_grid : maxgrid / 2, maxgrid));
     CoopGuard coop(c->device, block == 1024 ? std::max(grid, p.tail_live ? stage_g : 0u) : 0u,
                    c->num_cus);
     KC_HIP(hipEventRecord(g->ev0, c->stream), "hipEventRecord");
@@ -840,10 +843,12 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
         p.agg_live = agg_div == 0xffffffffu ? 0xffffffffu : Vn / agg_div;
         p.tail_live = tail_of(Vn);
         p.start_level = (uint32_t)sv[kc::KC_ST_TAIL_LEVEL];
+        p.agg_walks = 0;
         if (stage == 0) {
             // the compacted stages are latency-bound: fewer workgroups make
             // their barriers and shared-list polling cheaper (R-MAT s22 with
-            // 1024-thread groups: 3/8 of the CUs is the measured optimum)
+            // 1024-thread groups: 96 / 128 / 160 / 256 of 256 CUs take
+            // 11.76 / 11.67 / 11.71 / 12.14 ms)
             p.grid = stage_g;
         }
         if (p.trace) {

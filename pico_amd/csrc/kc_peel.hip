@@ -44,6 +44,7 @@ template <typename R, int M> struct PeelArgs {
     uint32_t *live; // live-vertex list (bucket: out; GPP: a buffer of its own), or null
     uint32_t live_thresh;
     uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
+    uint32_t agg_walks; // bucket: aggregate every walk's decrements (no pre-check)
     uint32_t tail_live;   // bucket: stop at a level start once <= this many are live
     uint32_t start_level; // bucket: first level (every vertex's degree is >= it)
     const uint32_t *abort_flag; // non-zero: the inputs are unusable, do nothing
@@ -321,8 +322,15 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
         // Otherwise skip targets whose claim bit is set, read with a plain
         // load (a stale word only lacks recent claims, so no live target is
         // skipped): a decrement of a claimed vertex is always undone, and
-        // these are most of the edges once the first levels are over.
+        // these are most of the edges once the first levels are over.  On
+        // the whole graph of a bucket peel that compacts (agg_walks), the
+        // decrements of the unclaimed targets are summed in LDS too: the
+        // sparse first levels aim most of them at a few hubs, whose words
+        // would serialise them at memory (Chung-Lu 2^24: 77 -> 59 ms); the
+        // compacted stages' dense levels lose more to the flush than the
+        // summing saves.
         const bool pc = L.precheck != 0;
+        const bool ag = pc || a.agg_walks != 0;
         uint64_t pre[WALK_U];
 #pragma unroll
         for (int k = 0; k < WALK_U; k++) {
@@ -347,7 +355,7 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
                 hit[k] = val[k] && (uint32_t)pre[k] > level;
                 // decrements to vertices well above the level are summed in
                 // LDS and applied once per walk (flush_agg)
-                if (pc && hit[k] && (uint32_t)pre[k] > level + KC_AGG_MARGIN && agg_add(L, u[k]))
+                if (ag && hit[k] && (uint32_t)pre[k] > level + KC_AGG_MARGIN && agg_add(L, u[k]))
                     hit[k] = false;
             }
             old[k] = hit[k] ? atomicSub(&a.deg[u[k]], 1u) : 0u;
@@ -1354,6 +1362,7 @@ static hipError_t launch_peel_m(const PeelLaunch &p, hipStream_t st)
     a.live = M == MODE_GPP ? p.live : p.out;
     a.live_thresh = M != MODE_PEELONE && a.live ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
+    a.agg_walks = M == MODE_BUCKET ? p.agg_walks : 0u;
     a.tail_live = M != MODE_GPP ? p.tail_live : 0u;
     a.abort_flag = p.abort_flag;
     a.start_level = M != MODE_GPP ? p.start_level : 0u;

@@ -233,7 +233,7 @@ void run_job(kc_server *s, kc_ctx *ctx, std::vector<Req *> &job, std::vector<kc_
     // by side instead of one after another; cores do not depend on the grid.
     kc_opts share{};
     share.grid = std::max(16u, ctx->num_cus / std::max(1u, s->o.slots));
-    share.stage_grid = std::min(share.grid, std::max(1u, ctx->num_cus * 3 / 8));
+    share.stage_grid = std::min(share.grid, std::max(1u, ctx->num_cus / 2));
     if (q0->r.kind == KC_REQ_DECOMPOSE_RMAT) {
         kc_graph *g = nullptr;
         st = kc_graph_gen_rmat(ctx, q0->r.scale, rmat_m_gen(q0->r), RMAT_A, RMAT_B, RMAT_C,
