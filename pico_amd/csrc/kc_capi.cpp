@@ -740,7 +740,7 @@ static kc_status run_peel(kc_graph *g, kc_method method, const kc_opts &o, kc_st
     p.start_level = 0;
     // the whole graph's walks sum their decrements in LDS when compacted
     // stages take over the dense levels (kc_peel.hip, walk_items)
-    p.agg_walks = mode == 0 && p.tail_live ? 1u : 0u;
+    p.agg_walks = mode != 2 && p.tail_live ? 1u : 0u;
     // workgroups of the compacted stages' launches (see below)
     const uint32_t stage_g =
         std::max(1u, std::min(o.stage_grid ? o.stage_grid : maxgrid / 2, maxgrid));

@@ -57,7 +57,7 @@ struct PeelLaunch {
     uint32_t *live; // GPP: its live-vertex list (V entries), or null
     uint32_t live_thresh; // bucket, GPP: scan a live-vertex list once <= this many remain
     uint32_t agg_live;    // pre-check degrees and aggregate hub decrements once <= this many remain
-    uint32_t agg_walks;   // bucket: sum every walk's decrements per workgroup in LDS (first stage)
+    uint32_t agg_walks;   // bucket, PeelOne: sum every walk's decrements per workgroup in LDS (first stage)
     uint32_t tail_live;   // bucket: stop at a level start once <= this many remain (0: never)
     uint32_t start_level; // bucket: first level to visit
     const uint32_t *abort_flag; // device word; non-zero: the kernel leaves at once (error 4)

@@ -44,7 +44,7 @@ template <typename R, int M> struct PeelArgs {
     uint32_t *live; // live-vertex list (bucket: out; GPP: a buffer of its own), or null
     uint32_t live_thresh;
     uint32_t agg_live; // bucket / PeelOne: pre-check + aggregate once <= this many live
-    uint32_t agg_walks; // bucket: aggregate every walk's decrements (no pre-check)
+    uint32_t agg_walks; // bucket / PeelOne: aggregate every walk's decrements (no pre-check)
     uint32_t tail_live;   // bucket: stop at a level start once <= this many are live
     uint32_t start_level; // bucket: first level (every vertex's degree is >= it)
     const uint32_t *abort_flag; // non-zero: the inputs are unusable, do nothing
@@ -323,12 +323,12 @@ __device__ void walk_items(PeelLds<BLOCK, LQ> &L, const PeelArgs<R, M> &a, uint3
         // load (a stale word only lacks recent claims, so no live target is
         // skipped): a decrement of a claimed vertex is always undone, and
         // these are most of the edges once the first levels are over.  On
-        // the whole graph of a bucket peel that compacts (agg_walks), the
+        // the whole graph of a peel that compacts (agg_walks), the
         // decrements of the unclaimed targets are summed in LDS too: the
         // sparse first levels aim most of them at a few hubs, whose words
-        // would serialise them at memory (Chung-Lu 2^24: 77 -> 59 ms); the
-        // compacted stages' dense levels lose more to the flush than the
-        // summing saves.
+        // would serialise them at memory (Chung-Lu 2^24: bucket peel 77 ->
+        // 59 ms, PeelOne 83 -> 65 ms); the compacted stages' dense levels
+        // lose more to the flush than the summing saves.
         const bool pc = L.precheck != 0;
         const bool ag = pc || a.agg_walks != 0;
         uint64_t pre[WALK_U];
@@ -1362,7 +1362,7 @@ static hipError_t launch_peel_m(const PeelLaunch &p, hipStream_t st)
     a.live = M == MODE_GPP ? p.live : p.out;
     a.live_thresh = M != MODE_PEELONE && a.live ? p.live_thresh : 0u;
     a.agg_live = p.agg_live;
-    a.agg_walks = M == MODE_BUCKET ? p.agg_walks : 0u;
+    a.agg_walks = M != MODE_GPP ? p.agg_walks : 0u;
     a.tail_live = M != MODE_GPP ? p.tail_live : 0u;
     a.abort_flag = p.abort_flag;
     a.start_level = M != MODE_GPP ? p.start_level : 0u;
