@@ -1770,18 +1770,26 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     if (a.nd_ds && cu)
                         e = s + wave_count_gt(a.nd_ds + s, d, cu - 1);
                     const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
-                    for (uint64_t i = s + lane; i < e; i += 4 * KC_WAVE) {
+                    // Only cnt < core is ever asked of cnt, and a count that
+                    // stops at cu is still a lower bound of at least cu: the
+                    // walk ends once cu neighbours at or above cu are seen
+                    // (the degree-ordered prefix holds the likeliest first).
+                    uint32_t tot = 0;
+                    for (uint64_t i0 = s; i0 < e; i0 += 4 * KC_WAVE) { // wave-uniform
                         uint32_t w[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
-                            uint64_t ii = i + (uint64_t)k * KC_WAVE;
+                            uint64_t ii = i0 + (uint64_t)k * KC_WAVE + lane;
                             w[k] = ii < e ? adj[ii] : 0xffffffffu;
                         }
 #pragma unroll
                         for (int k = 0; k < 4; k++)
                             c += w[k] != 0xffffffffu && a.core[w[k]] >= cu;
+                        tot = wave_sum_u32(c);
+                        if (tot >= cu)
+                            break;
                     }
-                    c = wave_sum_u32(c);
+                    c = tot;
                     if (lane == 0)
                         a.cnt[u] = c;
                     hpush(gs, phase, act_out, cap_list, lane == 0 && c < cu, u);
@@ -1793,18 +1801,22 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     if (a.nd_ds && cu) // the prefix of degree >= cu, as above (each wave alike)
                         e = s + wave_count_gt(a.nd_ds + s, (uint64_t)(e - s), cu - 1);
                     const uint32_t *adj = a.nd_ds ? a.col_ds : a.col;
-                    for (uint64_t i = s + threadIdx.x; i < e; i += 4 * (uint64_t)BLOCK) {
+                    uint32_t tot = 0; // stops at cu, as above
+                    for (uint64_t i0 = s; i0 < e; i0 += 4 * (uint64_t)BLOCK) { // block-uniform
                         uint32_t w[4];
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
-                            uint64_t ii = i + (uint64_t)k * BLOCK;
+                            uint64_t ii = i0 + (uint64_t)k * BLOCK + threadIdx.x;
                             w[k] = ii < e ? adj[ii] : 0xffffffffu;
                         }
 #pragma unroll
                         for (int k = 0; k < 4; k++)
                             c += w[k] != 0xffffffffu && a.core[w[k]] >= cu;
+                        tot = block_sum<BLOCK>(L, c);
+                        if (tot >= cu)
+                            break;
                     }
-                    c = block_sum<BLOCK>(L, c);
+                    c = tot;
                     if (threadIdx.x == 0)
                         a.cnt[u] = c;
                     if (wv == 0)
