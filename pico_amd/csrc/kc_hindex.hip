@@ -107,6 +107,7 @@ constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggrega
 // affected edge; updates aimed at the other vertices are patched as usual.
 constexpr uint32_t KC_PULL_T = 1024; // R-MAT s22: 512 29.8 ms, 1,024 28.2, 1,536 29.1
 constexpr uint32_t KC_PULL_DEN = 3; // default pull_den
+constexpr uint32_t KC_HOLD_DEN = 2; // NbrCore hold checks after rounds changing < 1/2 of their input
 constexpr int KC_PULL_U = 6;        // row entries in flight per lane in a hub pull (R-MAT s22: 2 27.2 ms, 4 26.6, 6 26.5, 8 26.7)
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
 
@@ -474,6 +475,41 @@ __device__ __forceinline__ uint32_t block_sum(HLds<BLOCK> &L, uint32_t x)
         t += L.red[i];
     __syncthreads();
     return t;
+}
+
+// NbrCore: whether v's value `old` (>= 1) holds, i.e. at least old neighbours
+// have a value >= old.  Only neighbours of degree >= old can (a value never
+// exceeds the degree), and the degree-ordered row holds them first, largest
+// degrees first; the count stops as soon as it reaches old, or once the
+// entries left could not reach it.  One wave; BLK: the whole workgroup.
+template <typename R, int BLOCK, bool BLK>
+__device__ bool value_holds(HLds<BLOCK> &L, const HArgs<R> &a, uint32_t v, uint32_t old)
+{
+    constexpr uint32_t GS = BLK ? BLOCK : KC_WAVE;
+    const uint32_t t = BLK ? threadIdx.x : lane_id();
+    const uint64_t s = a.rowptr[v];
+    const uint64_t e = s + wave_count_gt(a.nd_ds + s, (uint64_t)(a.rowptr[v + 1] - s), old - 1);
+    if (e - s < old)
+        return false;
+    uint32_t c = 0;
+    for (uint64_t i0 = s; i0 < e; i0 += 4 * (uint64_t)GS) { // uniform trip count
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint64_t ii = i0 + (uint64_t)k * GS + t;
+            w[k] = ii < e ? a.col_ds[ii] : 0xffffffffu;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            c += w[k] != 0xffffffffu && a.core[w[k]] >= old;
+        const uint32_t tot = BLK ? block_sum<BLOCK>(L, c) : wave_sum_u32(c);
+        if (tot >= old)
+            return true;
+        const uint64_t done = i0 + 4 * (uint64_t)GS < e ? i0 + 4 * (uint64_t)GS : e;
+        if (tot + (e - done) < old)
+            return false;
+    }
+    return false;
 }
 
 // Block-wide top-down search over HBM bins[1..top]; each thread reads four
@@ -1044,6 +1080,11 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
     uint32_t n_hubs = 0;    // HistoCore: listed hubs and their adjacency entries
     uint64_t hub_edges = 0;
     uint32_t n_giants = 0;  // listed vertices of more than KC_HBIG neighbours
+    // NbrCore: check that a marked vertex's value holds before rebuilding its
+    // histogram, in rounds after one where fewer than 1/KC_HOLD_DEN of the
+    // input changed (late rounds recompute tens of thousands of marked
+    // dense-core vertices of which a few hundred change)
+    bool hold_check = false;
     if (M == HM_HISTO) {
         if (a.hp->err) {
             if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -1325,6 +1366,9 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
             auto big_one = [&](uint32_t v) {
                 uint32_t old = a.core[v];
                 uint32_t sum;
+                if (M == HM_NBR && hold_check && old &&
+                    value_holds<R, BLOCK, true>(L, a, v, old))
+                    return; // every thread has the same answer
                 uint64_t b = hbase(a.rowptr, v);
                 block_hist_hyb<R, BLOCK>(&L.bins[0][0], a, v, old, a.histo + b, n_atom);
                 uint32_t k = block_hsearch<BLOCK>(L, a.histo + b, old, sum);
@@ -1390,6 +1434,9 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
                     if (!want(v) || (n_giants && d > KC_HBIG))
                         return; // listed ones were counted first
                     uint32_t old = a.core[v], sum;
+                    if (M == HM_NBR && hold_check && old &&
+                        value_holds<R, BLOCK, false>(L, a, v, old))
+                        return;
                     uint32_t k = wave_hindex<R, BLOCK>(L, a, v, old, sum, n_atom);
                     if (k == old) {
                         if (M == HM_CNT && lane == 0)
@@ -1436,6 +1483,8 @@ __global__ __launch_bounds__(BLOCK, M == HM_HISTO ? 4 : 1) void hindex_kernel(HA
         changed_total += n_changed;
         if (n_changed == 0)
             break;
+        if (M == HM_NBR)
+            hold_check = round > 1 && a.nd_ds && (uint64_t)n_changed * KC_HOLD_DEN < n_in;
 
         // ------------------------------------------------ update phase
         phase = gen;
