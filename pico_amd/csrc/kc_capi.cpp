@@ -406,6 +406,15 @@ kc_status kc_ctx_create(int device, kc_ctx **out)
         delete c;
         return hip_fail(e, "context stream / events");
     }
+    // every kernel loaded now, not by the first decomposition that needs it
+    // (the degree-ordered adjacency's sort alone: ~19 ms once per process)
+    if ((e = kc::load_modules(c->stream)) != hipSuccess) {
+        hipEventDestroy(c->ev1);
+        hipEventDestroy(c->ev0);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return hip_fail(e, "kernel load");
+    }
     // the no-allocation entry point's scratch array, located once per device
     // (a failure is reported by that entry point, not here)
     noalloc_setup(device);
@@ -948,10 +957,12 @@ static kc_status run_hindex(kc_graph *g, kc_method method, const kc_opts &o, kc_
              (s = ensure(g, g->hb_ds, nb, "neighbour histogram bases"))))
             return s;
         double tp = now_ms();
-        // the histogram buffer (>= nnz words) is free scratch before the run
+        // the histogram buffer (nnz + V + 1 words) is free scratch before the
+        // run, and so is the neighbour-base array, filled after the sort
         hipError_t pe = kc::build_degree_sorted(g->rowptr, g->wide, g->col, g->V, g->nnz,
                                                 (uint32_t *)g->col_ds.p, (uint32_t *)g->nd_ds.p,
-                                                (uint32_t *)g->histo.p, c->stream);
+                                                (uint32_t *)g->histo.p,
+                                                (uint32_t *)g->hb_ds.p, c->stream);
         if (pe == hipSuccess && g->nnz + g->V < 0xffffffffull)
             pe = kc::build_nbr_hbase(g->rowptr, g->wide, (const uint32_t *)g->col_ds.p, g->nnz,
                                      (uint32_t *)g->hb_ds.p, c->stream);

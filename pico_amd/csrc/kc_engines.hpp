@@ -265,11 +265,34 @@ int batch_occupancy(uint32_t block);
 uint32_t batch_lds_vertices(uint32_t block);
 
 // Rows re-ordered by neighbour degree, largest first (kc_prep.hip); nd_out
-// receives the neighbour degrees; nd_tmp is nnz words of scratch.
+// receives the neighbour degrees.
 hipError_t build_nbr_hbase(const void *rowptr, bool wide, const uint32_t *adj, uint64_t nnz,
                            uint32_t *hb, hipStream_t st);
+// nd_tmp: nnz + 1 words of scratch; col_tmp: nnz words of scratch, or null
+// (the sort then allocates its own copy of the keys and values).
 hipError_t build_degree_sorted(const void *rowptr, bool wide, const uint32_t *col, uint32_t V,
                                uint64_t nnz, uint32_t *col_out, uint32_t *nd_out,
-                               uint32_t *nd_tmp, hipStream_t st);
+                               uint32_t *nd_tmp, uint32_t *col_tmp, hipStream_t st);
+
+// An empty launch from each source file's code object (kc_*.hip), and all of
+// them in turn: loads every kernel of the library.
+hipError_t peel_touch(hipStream_t st);
+hipError_t compact_touch(hipStream_t st);
+hipError_t hindex_touch(hipStream_t st);
+hipError_t prep_touch(hipStream_t st);
+hipError_t batch_touch(hipStream_t st);
+hipError_t gen_touch(hipStream_t st);
+hipError_t dyn_touch(hipStream_t st);
+inline hipError_t load_modules(hipStream_t st)
+{
+    hipError_t (*const fns[])(hipStream_t) = {peel_touch, compact_touch, hindex_touch, prep_touch,
+                                              batch_touch, gen_touch, dyn_touch};
+    for (auto f : fns) {
+        hipError_t e = f(st);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipStreamSynchronize(st);
+}
 
 } // namespace kc
