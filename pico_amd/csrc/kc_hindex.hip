@@ -31,7 +31,7 @@ namespace kc {
 
 enum { HM_HISTO = 0, HM_CNT = 1, HM_NBR = 2 };
 
-constexpr uint32_t KC_LBINS = 512;  // LDS histogram bins per wavefront
+constexpr uint32_t KC_LBINS = 1024; // LDS histogram bins per wavefront (R-MAT s22 HistoCore: 512 26.5 ms, 1,024 25.9)
 constexpr uint32_t KC_HBIG = 4096;  // items above this go to the workgroup
 // HistoCore: vertices of at most this many neighbours get a thread each (not a
 // wave) for their initial histogram and their first count

@@ -144,15 +144,15 @@ def _clique_with_leaves(n, leaves):
     return rowptr, dst[order].astype(np.uint32)
 
 
-@pytest.mark.parametrize("n,block,grid", [(1100, 512, 0), (1100, 512, 3), (2100, 256, 0)])
+@pytest.mark.parametrize("n,block,grid", [(1100, 512, 0), (1100, 512, 3), (4200, 256, 0)])
 def test_histocore_hub_rebuild_paths(ctx, n, block, grid):
     # a round that reaches many hub edges lets every hub (more than 1,024
     # neighbours) pull its changed neighbours' updates instead of receiving
-    # patches: cores of 1,099 go to the workgroup (LDS bins), a workgroup's
-    # overflow of such hubs (grid 3) to one wave with HBM atomics for the bins
-    # from 512 up, and 2099 (above a 256-thread group's LDS bins) to HBM
-    # atomics; bin 0 and the bins above the core must come out as the
-    # reference leaves them
+    # patches: cores of 1,099 (above a wave's 1,024 LDS bins) go to the
+    # workgroup, a workgroup's overflow of such hubs (grid 3) to one wave with
+    # HBM atomics for the bins from 1,024 up, and 4199 (above a 256-thread
+    # group's 4,096 LDS bins) to HBM atomics; bin 0 and the bins above the
+    # core must come out as the reference leaves them
     rp, col = _clique_with_leaves(n, 10)
     _, st = _check(ctx, rp, col, "histocore", hist=True, block=block, grid=grid)
     assert st["rebuild_rounds"] > 0, st
@@ -171,8 +171,8 @@ def _union(*parts):
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_histocore_wave_rebuild_and_hub_aggregation(ctx, block):
     # K_100 whose members carry 1,100 leaves each: hubs of 1,199 neighbours
-    # (above the 1,024 of a pull hub, at most 4,096) with core 99 (below the
-    # 512 LDS bins), pulled by one wave each when the first round's changed
+    # (above the 1,024 of a pull hub, at most 4,096) with core 99 (below a
+    # wave's 1,024 LDS bins), pulled by one wave each when the first round's changed
     # hubs reach all of the hub adjacency; beside it K_300 with 50 leaves
     # each: 349 neighbours (257..512), whose bin updates are summed per
     # workgroup in LDS instead of patched one atomic at a time
