@@ -306,9 +306,16 @@ typedef struct kc_batch_info {
 /* Launch options of the batched engine; zero fields select the autotuner's
  * pick when kc_batch_autotune has run, else defaults. */
 typedef struct kc_batch_opts {
-    uint32_t block;  /* threads per workgroup: 256, 512 or 1024 */
-    uint32_t per_cu; /* workgroups launched per CU */
-    uint32_t reserved[6];
+    uint32_t block;        /* threads per workgroup: 256, 512 or 1024 */
+    uint32_t per_cu;       /* workgroups launched per CU */
+    uint32_t lds_vertices; /* graphs of at most this many vertices keep their
+                              degrees in LDS: 4096 or 8192 (block 256), 8192 or
+                              16384 (512), 16384 or 32768 (1024); default the
+                              larger.  Larger graphs use HBM. */
+    uint32_t split;        /* cascade edges per thread per pass, the width of
+                              the edge-balanced split of a level's frontier: 4
+                              or 8 (default) */
+    uint32_t reserved[4];
 } kc_batch_opts;
 
 typedef struct kc_batch_stats {
@@ -354,7 +361,8 @@ KC_API kc_status kc_batch_run(kc_batch *b, const kc_batch_opts *opts, uint32_t *
 KC_API kc_status kc_batch_run_plain(kc_batch *b, kc_method method, const kc_opts *opts,
                                     uint32_t *core_out, uint32_t *kmax_out,
                                     kc_batch_stats *stats);
-/* Time every (block, per_cu) candidate on the batch itself, keep the fastest
+/* Time every (block, per_cu) candidate on the batch itself, then every
+ * (lds_vertices, split) candidate at the fastest of those; keep the fastest
  * for later kc_batch_run calls and return it. */
 KC_API kc_status kc_batch_autotune(kc_batch *b, uint32_t reps, kc_batch_opts *best,
                                    double *best_ms);

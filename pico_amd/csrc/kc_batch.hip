@@ -24,7 +24,6 @@
 
 namespace kc {
 
-static constexpr int BU = 8; // cascade edges per thread per pass
 
 template <int BLOCK, int LDS_V> struct BLds {
     uint32_t deg[LDS_V];
@@ -115,7 +114,8 @@ __device__ __forceinline__ void b_push(BLds<BLOCK, LDS_V> &L, uint32_t *q, bool 
         q[base + lane_rank(m)] = v;
 }
 
-template <int BLOCK, int LDS_V>
+// BU: cascade edges per thread per pass (the edge-balanced split width).
+template <int BLOCK, int LDS_V, int BU>
 __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchArgs a)
 {
     __shared__ BLds<BLOCK, LDS_V> L;
@@ -274,36 +274,59 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(BatchArgs a)
     }
 }
 
-template <int BLOCK, int LDS_V> static hipError_t launch_batch_t(const BatchArgs &a, uint32_t grid,
-                                                                  hipStream_t st)
+template <int BLOCK, int LDS_V, int BU>
+static hipError_t launch_batch_t(const BatchArgs &a, uint32_t grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((batch_kernel<BLOCK, LDS_V>), dim3(grid), dim3(BLOCK), 0, st, a);
+    hipLaunchKernelGGL((batch_kernel<BLOCK, LDS_V, BU>), dim3(grid), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 
-int batch_occupancy(uint32_t block)
+template <int BLOCK, int LDS_V, int BU> static int occupancy_t()
 {
     int nb = 0;
-    hipError_t e = hipErrorInvalidValue;
-    switch (block) {
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<256, 8192>, 256, 0); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<512, 16384>, 512, 0); break;
-    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<1024, 16384>, 1024, 0); break;
-    default: break;
-    }
-    return e == hipSuccess ? nb : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<BLOCK, LDS_V, BU>, BLOCK,
+                                                        0) == hipSuccess
+               ? nb
+               : 0;
 }
 
-uint32_t batch_lds_vertices(uint32_t block) { return block == 256 ? 8192u : 16384u; }
+// The built shapes: per block size a small and a large LDS degree capacity
+// (the large one is the default), each with 4 or 8 cascade edges per thread
+// (8 the default).  Dispatch by the three parameters; 0 when not built.
+#define KC_BATCH_SHAPES(X)                                                                        \
+    X(256, 4096, 4) X(256, 4096, 8) X(256, 8192, 4) X(256, 8192, 8)                               \
+    X(512, 8192, 4) X(512, 8192, 8) X(512, 16384, 4) X(512, 16384, 8)                             \
+    X(1024, 16384, 4) X(1024, 16384, 8) X(1024, 32768, 4) X(1024, 32768, 8)
 
-hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st)
+int batch_occupancy(uint32_t block, uint32_t lds_v, uint32_t bu)
+{
+#define KC_OCC(B, V, U)                                                                           \
+    if (block == B && lds_v == V && bu == U)                                                      \
+        return occupancy_t<B, V, U>();
+    KC_BATCH_SHAPES(KC_OCC)
+#undef KC_OCC
+    return 0;
+}
+
+uint32_t batch_lds_vertices(uint32_t block, bool large)
 {
     switch (block) {
-    case 256: return launch_batch_t<256, 8192>(a, grid, st);
-    case 512: return launch_batch_t<512, 16384>(a, grid, st);
-    case 1024: return launch_batch_t<1024, 16384>(a, grid, st);
-    default: return hipErrorInvalidValue;
+    case 256: return large ? 8192u : 4096u;
+    case 512: return large ? 16384u : 8192u;
+    case 1024: return large ? 32768u : 16384u;
+    default: return 0;
     }
+}
+
+hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t lds_v, uint32_t bu,
+                        uint32_t grid, hipStream_t st)
+{
+#define KC_LAUNCH(B, V, U)                                                                        \
+    if (block == B && lds_v == V && bu == U)                                                      \
+        return launch_batch_t<B, V, U>(a, grid, st);
+    KC_BATCH_SHAPES(KC_LAUNCH)
+#undef KC_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 // The runtime loads a source file's kernels at the first launch of any of

@@ -32,6 +32,11 @@ struct kc_batch {
 
 namespace {
 
+// one built shape of the batch kernel (kc_batch.hip)
+struct BatchShape {
+    uint32_t block, lds_v, bu;
+};
+
 template <typename T> hipError_t dalloc(kc_batch *b, T **p, size_t n)
 {
     size_t bytes = n ? n * sizeof(T) : sizeof(T);
@@ -86,12 +91,24 @@ uint32_t first_bad_graph(uint32_t G, const uint64_t *vo, const uint64_t *eo, con
     return bad.load();
 }
 
-kc_status batch_grid(const kc_batch *b, const kc_batch_opts &o, uint32_t &block, uint32_t &grid)
+kc_status batch_grid(const kc_batch *b, const kc_batch_opts &o, BatchShape &sh, uint32_t &grid)
 {
-    block = o.block ? o.block : (b->tuned.block ? b->tuned.block : 512);
+    const uint32_t block = o.block ? o.block : (b->tuned.block ? b->tuned.block : 512);
     if (block != 256 && block != 512 && block != 1024)
         return fail(KC_EINVAL, "block must be 256, 512 or 1024");
-    int occ = kc::batch_occupancy(block);
+    // the tuned shape's capacity and split apply to its own block size only
+    const bool tb = b->tuned.block == block;
+    uint32_t lv = o.lds_vertices ? o.lds_vertices
+                                 : (tb && b->tuned.lds_vertices ? b->tuned.lds_vertices
+                                                                : kc::batch_lds_vertices(block, true));
+    uint32_t bu = o.split ? o.split : (tb && b->tuned.split ? b->tuned.split : 8u);
+    if (lv != kc::batch_lds_vertices(block, true) && lv != kc::batch_lds_vertices(block, false))
+        return fail(KC_EINVAL, "lds_vertices %u is not built for block %u (%u or %u)", lv, block,
+                    kc::batch_lds_vertices(block, false), kc::batch_lds_vertices(block, true));
+    if (bu != 4 && bu != 8)
+        return fail(KC_EINVAL, "split must be 4 or 8");
+    sh = BatchShape{block, lv, bu};
+    int occ = kc::batch_occupancy(block, lv, bu);
     if (occ <= 0)
         return fail(KC_EDEVICE, "batch kernel cannot be resident with block %u", block);
     uint32_t per_cu = o.per_cu ? o.per_cu : (b->tuned.per_cu ? b->tuned.per_cu : (uint32_t)occ);
@@ -102,7 +119,7 @@ kc_status batch_grid(const kc_batch *b, const kc_batch_opts &o, uint32_t &block,
 }
 
 // one launch over the whole batch; results stay on the device
-kc_status batch_launch(kc_batch *b, uint32_t block, uint32_t grid, float &ms)
+kc_status batch_launch(kc_batch *b, const BatchShape &sh, uint32_t grid, float &ms)
 {
     hipStream_t st = b->ctx->stream;
     kc::BatchArgs a{};
@@ -120,7 +137,7 @@ kc_status batch_launch(kc_batch *b, uint32_t block, uint32_t grid, float &ms)
     a.levels = b->d_levels;
     KC_HIP(hipMemsetAsync(b->d_counter, 0, 4, st), "counter clear");
     KC_HIP(hipEventRecord(b->ev0, st), "hipEventRecord");
-    hipError_t e = kc::launch_batch(a, block, grid, st);
+    hipError_t e = kc::launch_batch(a, sh.block, sh.lds_v, sh.bu, grid, st);
     if (e != hipSuccess)
         return hip_fail(e, "batch launch");
     KC_HIP(hipEventRecord(b->ev1, st), "hipEventRecord");
@@ -840,17 +857,18 @@ kc_status kc_batch_run(kc_batch *b, const kc_batch_opts *opts, uint32_t *core_ou
     kc_batch_opts o{};
     if (opts)
         o = *opts;
-    uint32_t block = 0, grid = 0;
-    kc_status s = batch_grid(b, o, block, grid);
+    BatchShape sh{};
+    uint32_t grid = 0;
+    kc_status s = batch_grid(b, o, sh, grid);
     if (s)
         return s;
     kc_batch_stats st{};
     float ms = 0;
-    if (b->G && (s = batch_launch(b, block, grid, ms)))
+    if (b->G && (s = batch_launch(b, sh, grid, ms)))
         return s;
     if ((s = batch_results(b, core_out, kmax_out, st, true)))
         return s;
-    st.block = block;
+    st.block = sh.block;
     st.grid = grid;
     st.launches = b->G ? 1 : 0;
     st.device_ms = ms;
@@ -930,33 +948,65 @@ kc_status kc_batch_autotune(kc_batch *b, uint32_t reps, kc_batch_opts *best, dou
     KC_HIP(hipSetDevice(b->ctx->device), "hipSetDevice");
     if (!reps)
         reps = 1;
+    const kc_batch_opts keep = b->tuned;
+    b->tuned = kc_batch_opts{}; // candidates are timed with their own fields only
     kc_batch_opts bo{};
     double bms = 1e300;
+    kc_status s = KC_OK;
+    auto time_one = [&](const kc_batch_opts &o) -> kc_status {
+        BatchShape sh{};
+        uint32_t grid = 0;
+        kc_status r = batch_grid(b, o, sh, grid);
+        if (r)
+            return r;
+        double t = 1e300;
+        for (uint32_t k = 0; k < reps && b->G; k++) {
+            float ms = 0;
+            if ((r = batch_launch(b, sh, grid, ms)))
+                return r;
+            t = std::min(t, (double)ms);
+        }
+        if (t < bms) {
+            bms = t;
+            bo = o;
+        }
+        return KC_OK;
+    };
+    // block x workgroups per CU at the default capacity and split ...
     for (uint32_t block : {256u, 512u, 1024u}) {
-        int occ = kc::batch_occupancy(block);
+        int occ = kc::batch_occupancy(block, kc::batch_lds_vertices(block, true), 8);
         for (uint32_t per_cu : {1u, 2u, 4u, 8u}) {
             if (per_cu > (uint32_t)occ)
                 continue;
             kc_batch_opts o{};
             o.block = block;
             o.per_cu = per_cu;
-            uint32_t blk = 0, grid = 0;
-            kc_status s = batch_grid(b, o, blk, grid);
-            if (s)
-                return s;
-            double t = 1e300;
-            for (uint32_t r = 0; r < reps && b->G; r++) {
-                float ms = 0;
-                if ((s = batch_launch(b, blk, grid, ms)))
-                    return s;
-                t = std::min(t, (double)ms);
-            }
-            if (t < bms) {
-                bms = t;
-                bo = o;
-            }
+            if ((s = time_one(o)))
+                break;
         }
+        if (s)
+            break;
     }
+    // ... then the LDS degree capacity and the cascade split at the best of them
+    if (!s) {
+        const kc_batch_opts base = bo;
+        for (bool large : {true, false})
+            for (uint32_t split : {8u, 4u}) {
+                kc_batch_opts o = base;
+                o.lds_vertices = kc::batch_lds_vertices(base.block, large);
+                o.split = split;
+                if (!s)
+                    s = time_one(o);
+            }
+    }
+    if (s) {
+        b->tuned = keep;
+        return s;
+    }
+    if (!bo.lds_vertices)
+        bo.lds_vertices = kc::batch_lds_vertices(bo.block, true);
+    if (!bo.split)
+        bo.split = 8;
     b->tuned = bo;
     if (best)
         *best = bo;

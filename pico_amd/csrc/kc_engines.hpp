@@ -260,9 +260,13 @@ hipError_t copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t st
 // order-independent checksum of a core vector: sum of mix64(v << 32 | core[v])
 hipError_t core_checksum(const uint32_t *core, uint32_t V, uint64_t *out, hipStream_t st);
 
-hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t grid, hipStream_t st);
-int batch_occupancy(uint32_t block);
-uint32_t batch_lds_vertices(uint32_t block);
+// The batched engine's launch shapes: block 256 / 512 / 1024 threads, LDS
+// degree capacity batch_lds_vertices(block, large) vertices, bu = 4 or 8
+// cascade edges per thread per pass.  Occupancy 0: not a built shape.
+hipError_t launch_batch(const BatchArgs &a, uint32_t block, uint32_t lds_v, uint32_t bu,
+                        uint32_t grid, hipStream_t st);
+int batch_occupancy(uint32_t block, uint32_t lds_v, uint32_t bu);
+uint32_t batch_lds_vertices(uint32_t block, bool large);
 
 // Rows re-ordered by neighbour degree, largest first (kc_prep.hip); nd_out
 // receives the neighbour degrees.

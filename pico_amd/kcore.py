@@ -100,7 +100,8 @@ class BatchInfo(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("per_cu", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+                ("lds_vertices", ctypes.c_uint32), ("split", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class BatchStats(ctypes.Structure):
@@ -716,9 +717,9 @@ class DeviceBatch:
     def device_bytes(self):
         return int(lib().kc_batch_device_bytes(self._h))
 
-    def run(self, block=0, per_cu=0, copy=True):
+    def run(self, block=0, per_cu=0, copy=True, lds_vertices=0, split=0):
         o = BatchOpts()
-        o.block, o.per_cu = block, per_cu
+        o.block, o.per_cu, o.lds_vertices, o.split = block, per_cu, lds_vertices, split
         st = BatchStats()
         core = np.zeros(max(self.vertices, 1), np.uint32) if copy else None
         km = np.zeros(max(self.G, 1), np.uint32) if copy else None
@@ -749,7 +750,8 @@ class DeviceBatch:
         o = BatchOpts()
         ms = ctypes.c_double(0)
         _check(lib().kc_batch_autotune(self._h, reps, ctypes.byref(o), ctypes.byref(ms)))
-        return {"block": o.block, "per_cu": o.per_cu}, ms.value
+        return {"block": o.block, "per_cu": o.per_cu, "lds_vertices": o.lds_vertices,
+                "split": o.split}, ms.value
 
     def close(self):
         if self._h:

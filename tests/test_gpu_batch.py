@@ -61,6 +61,35 @@ def test_batch_launch_shapes(ctx, block, per_cu):
     db.close()
 
 
+LDS_SIZES = {256: (4096, 8192), 512: (8192, 16384), 1024: (16384, 32768)}
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+@pytest.mark.parametrize("large", [False, True])
+@pytest.mark.parametrize("split", [4, 8])
+def test_batch_lds_capacity_and_split(ctx, block, large, split):
+    # graphs on both sides of each LDS degree capacity (2^12 .. 2^15
+    # vertices) with small ones; every built shape gives the same cores
+    gs = [pa.HostGraph.rmat(s, 6, seed=s) for s in (12, 13, 14, 15)]
+    gs += [pa.HostGraph.rmat(8, 4, seed=7), pa.HostGraph.er(5000, 20000, seed=2)]
+    hb = pa.HostBatch.from_graphs(gs)
+    db = pa.DeviceBatch(ctx, hb)
+    core, km, st = db.run(block=block, lds_vertices=LDS_SIZES[block][large], split=split)
+    assert st["block"] == block
+    _check_batch(hb, core, km, st["levels_run"])
+    db.close()
+
+
+def test_batch_shape_validation(ctx):
+    hb = pa.HostBatch.rmat(4, 5, 7, seed=1)
+    db = pa.DeviceBatch(ctx, hb)
+    with pytest.raises(pa.KCoreError):
+        db.run(block=512, lds_vertices=32768)  # built for 1024 only
+    with pytest.raises(pa.KCoreError):
+        db.run(split=6)
+    db.close()
+
+
 def test_batch_graphs_beyond_lds(ctx):
     """Graphs with more vertices than a workgroup keeps in LDS (>16384) take
     the HBM-resident degree path; mixed with small ones."""
@@ -129,6 +158,7 @@ def test_batch_autotune_keeps_results(ctx):
     db = pa.DeviceBatch(ctx, hb)
     best, ms = db.autotune(reps=1)
     assert best["block"] in (256, 512, 1024) and best["per_cu"] >= 1 and ms > 0
+    assert best["lds_vertices"] in LDS_SIZES[best["block"]] and best["split"] in (4, 8)
     core, km, st = db.run()
     assert st["block"] == best["block"]
     _check_batch(hb, core, km, st["levels_run"])
