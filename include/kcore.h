@@ -95,7 +95,7 @@ typedef struct kc_opts {
                             stages (<= occupancy * CUs); 0: half of the largest resident grid */
     uint32_t pull_div;   /* HistoCore: a round applies the hubs' updates by pulls (each hub
                             scans its row against the round's changed vertices) when the
-                            round's walk exceeds 1/pull_div of the hubs' adjacency; 0: 3,
+                            round's walk exceeds 1/pull_div of the hubs' adjacency; 0: 5,
                             0xffffffff: always */
 } kc_opts;
 

@@ -106,7 +106,7 @@ constexpr uint32_t KC_HUB_AGG = 256; // neighbours above this degree are aggrega
 // vertices and sums the updates in LDS -- instead of receiving two atomics per
 // affected edge; updates aimed at the other vertices are patched as usual.
 constexpr uint32_t KC_PULL_T = 1024; // R-MAT s22: 512 29.8 ms, 1,024 28.2, 1,536 29.1
-constexpr uint32_t KC_PULL_DEN = 3; // default pull_den
+constexpr uint32_t KC_PULL_DEN = 5; // default pull_den (R-MAT s22: 2 / 3 / 4 / 5 / 6: 26.9 / 26.0 / 25.9 / 25.7 / 25.7 ms)
 constexpr uint32_t KC_HOLD_DEN = 2; // NbrCore hold checks after rounds changing < 1/2 of their input
 constexpr int KC_PULL_U = 6;        // row entries in flight per lane in a hub pull (R-MAT s22: 2 27.2 ms, 4 26.6, 6 26.5, 8 26.7)
 constexpr uint32_t KC_HEMPTY = 0xffffffffu;
