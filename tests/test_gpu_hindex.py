@@ -199,3 +199,15 @@ def test_histocore_pull_rounds_match_patches(ctx, pull_div, graph):
     _, st = _check(ctx, hg.rowptr, hg.col, "histocore", hist=True, pull_div=pull_div)
     if pull_div == 0xFFFFFFFF:
         assert st["rebuild_rounds"] > 0, st
+
+
+@pytest.mark.parametrize("leaves", [255, 256, 65535, 65536])
+def test_degree_sort_byte_boundaries(ctx, leaves):
+    # the degree-ordered rows sort only the bytes the largest degree needs: a
+    # hub of 255 / 256 / 65,535 / 65,536 neighbours (one, two or three key
+    # bytes) beside a clique whose members carry leaves; the rows must come
+    # out ordered either way
+    rp, col = _union(_clique_with_leaves(1, leaves), _clique_with_leaves(40, 3),
+                     _clique_with_leaves(12, 200))
+    for method in ("histocore", "nbrcore"):
+        _check(ctx, rp, col, method, hist=(method == "histocore"))
