@@ -77,7 +77,8 @@ typedef struct kc_opts {
     uint32_t grid;       /* persistent workgroups (<= occupancy * CUs); 0: the method's
                             default, narrowed for graphs of fewer than ~30M adjacency
                             entries to about 48 * sqrt(nnz / 1e6), at least 32 */
-    uint32_t hub_split;  /* edges per virtual node of a hub vertex */
+    uint32_t hub_split;  /* edges per virtual node of a hub vertex; 0: 4096 (kc_decompose_noalloc:
+                            1024) */
     uint32_t queue_cap;  /* shared work-list capacity in items */
     uint32_t gpp_compat; /* KC_GPP: report isolated vertices as 1 (reference quirk) */
     uint32_t part_size;  /* PeelOne reference part width (reporting only) */
